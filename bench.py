@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""PBF solver step benchmark on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--workload dam_break_262k] [--no-cpu]
+
+A step is one pbf_step of the whole solver (predict, hash, counting sort + scan, reorder,
+neighbour build, I Jacobi iterations, curl, vorticity + XSPH) over the synthetic dam-break
+state, resident in HBM.  W untimed steps, then K steps timed with HIP events recorded on the
+solver's stream (hipGraph replay).  Prints ONE JSON line.
+
+roofline: the dominant kernel group of an eagerly launched step (HIP events on the solver's
+stream around it), with its algorithmic bytes per launch (fp32 compulsory traffic; neighbour
+gathers counted as cache hits).  cpu_baseline: the oracle's C++ restatement of the reference
+step on a bounded sample of the same workload, all host threads.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from pbf_amd import FLAG_GRAPH, Solver, scenes  # noqa: E402
+
+METRIC = "ms/step + particle-steps/s @262k; time to 0.5% density err, learned vs plain"
+HBM_PEAK_GBS = 8000.0
+# Reference's published figure: 262144 particles, 0.012-0.015 s per cudaPBFSolve on an RTX 3090
+# (README.md:9); the faster end as particle-steps/s.
+PUBLISHED_PSPS = 262144 / 0.012
+
+WORKLOADS = {
+    "dam_break_262k": scenes.config_262k,
+    "cpu_32k": scenes.config_cpu_32k,
+    "double_dam_break_1m": scenes.config_1m,
+}
+
+
+def algorithmic_bytes(stage: str, n: int, ncells: int) -> float:
+    """Compulsory fp32 bytes per launch group (12 B per xyz stream, 4 B per scalar)."""
+    per_particle = {
+        "predict_hash": 64,  # R x, v; W x_pred, v, cell
+        "scan": 0,  # per cell, below
+        "scatter_rank_reorder": 76,  # R cell id, 3 gathered xyz streams; W 3 xyz streams, id
+        "neighbors_lambda": 16,  # R x; W lambda
+        "lambda": 16,  # R x; W lambda
+        "delta_apply": 28,  # R x, lambda; W x
+        "curl": 36,  # R x, v; W curl
+        "vort_apply": 48,  # R x, v, curl; W v
+    }
+    b = per_particle.get(stage, 0) * n
+    if stage == "scan":
+        b += 12 * ncells  # R count, W start, clear count
+    return float(b)
+
+
+def step_bytes(n: int, ncells: int, iters: int) -> float:
+    """Whole-step algorithmic bytes: N*(252 + 44*I) + 12*C."""
+    return float(n * (252 + 44 * iters) + 12 * ncells)
+
+
+def cpu_baseline(scene, budget_s: float = 12.0):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle, orc  # noqa: E402
+
+    threads = orc().orc_max_threads()
+    o = Oracle.from_scene(scene)
+    o.step(1)  # warm-up
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        o.step(1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 200:
+            break
+    psps = scene.n * steps / el
+    return {
+        "value": psps,
+        "unit": "particle-steps/s",
+        "cores": int(threads),
+        "kind": "port",
+        "sample": f"{steps} steps of {scene.name} ({scene.n} particles) after 1 warm-up step, {el:.1f} s",
+        "ms_per_step": 1000.0 * el / steps,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--workload", default="dam_break_262k", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(
+            f"bench.py runs on one GPU only: the solver path shards by spatial slabs in design, "
+            f"but no multi-GPU run is built; got --gpus {args.gpus}",
+            file=sys.stderr,
+        )
+        sys.exit(2)
+
+    scene = WORKLOADS[args.workload]()
+    flags = 0 if args.no_graph else FLAG_GRAPH
+    s = Solver.from_scene(scene, flags=flags)
+    n = scene.n
+    ncells = int(np.prod(scene.grid_dims))
+    iters = int(scene.params["iterations"])
+
+    if args.warmup > 0:
+        s.advance(args.warmup)
+        s.synchronize()
+    ms_total = s.time_steps(args.steps)
+    ms_step = ms_total / max(args.steps, 1)
+    value = n / (ms_step * 1e-3)
+
+    # per-kernel-group times of eagerly launched steps (HIP events on the solver stream)
+    acc = {}
+    cnt = {}
+    for _ in range(max(1, args.profile_steps)):
+        for name, ms in s.profile_step():
+            acc[name] = acc.get(name, 0.0) + ms
+            cnt[name] = cnt.get(name, 0) + 1
+    launches_per_step = {k: cnt[k] / max(1, args.profile_steps) for k in cnt}
+    avg_ms = {k: acc[k] / cnt[k] for k in acc}  # per launch
+    per_step_ms = {k: avg_ms[k] * launches_per_step[k] for k in avg_ms}
+    dom = max(per_step_ms, key=per_step_ms.get)
+    dom_bytes = algorithmic_bytes(dom, n, ncells)
+    achieved = dom_bytes / (avg_ms[dom] * 1e-3) / 1e9
+    sort_stage = ["predict_hash", "scan", "scatter_rank_reorder"]
+    sort_ms = sum(per_step_ms.get(k, 0.0) for k in sort_stage)
+    sort_bytes = float(n * 96 + 12 * ncells)
+
+    pos, vel = s.get_state()
+    finite = bool(np.isfinite(pos).all() and np.isfinite(vel).all())
+    s.close()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(scene)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / PUBLISHED_PSPS,
+        "dtype": "f32",
+        "data": "synthetic (seeded SplitMix64 dam-break lattice)",
+        "config": {
+            "workload": scene.name,
+            "particles": n,
+            "grid": list(scene.grid_dims),
+            "iterations": iters,
+            "dt": scene.params["dt"],
+            "graph": not args.no_graph,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": dom_bytes,
+            "avg_launch_ms": avg_ms[dom],
+        },
+        "stages_ms": {k: round(v, 5) for k, v in per_step_ms.items()},
+        "step_roofline_frac": step_bytes(n, ncells, iters) / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9),
+        "sort_reorder": {"ms": sort_ms, "frac": sort_bytes / (sort_ms * 1e-3) / (HBM_PEAK_GBS * 1e9) if sort_ms else None},
+        "state_finite": finite,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,170 @@
+/* pbf.h -- C ABI of the MI355X (gfx950) Position Based Fluids solver (libpbf_hip.so).
+ *
+ * Drop-in boundary for the reference's solver host API, kernels/cudaPBF.h:8-12
+ * (initCUDASolverDataBuffer / cudaPBFSolve / freeCUDASolverDataBuffer, a C++ ABI taking
+ * std::vector<glm::vec3>, SolverData& and SolverParameter&).  Here every entry point is
+ * extern "C" with plain pointers and sizes.  A header-only C++ facade that re-creates the
+ * reference's function names and its PBFSolver class on top of this ABI is in
+ * include/pbf_solver.hpp.
+ *
+ * Conventions
+ *   - Every function returns PBF_OK (0) or a negative PBF_ERR_* code; pbf_last_error()
+ *     describes the last failure of the calling thread.  Nothing calls exit() (the reference's
+ *     CHECK_CUDA does, kernels/cudaPBFSolver.cuh:5-10).
+ *   - Host arrays of positions / velocities are float xyz triples (AoS, 12 bytes per particle)
+ *     in the caller's particle order, as the reference's SolverData.position / .velocity
+ *     (include/solverBase.h:45-47).  Inside, the solver keeps cell-sorted SoA-of-float4.
+ *   - One solver = one HIP stream on the current device.  pbf_step / pbf_solve only enqueue
+ *     work; functions returning host data synchronise that stream.  Solvers are independent.
+ */
+#ifndef PBF_H_
+#define PBF_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PBF_OK 0
+#define PBF_ERR_ARG (-1)      /* bad argument */
+#define PBF_ERR_HIP (-2)      /* a HIP runtime call failed */
+#define PBF_ERR_CAPACITY (-3) /* size beyond what the solver was created for */
+#define PBF_ERR_STATE (-4)    /* call not valid in the solver's current state */
+#define PBF_ERR_NOGPU (-5)    /* no HIP device */
+
+/* flags */
+#define PBF_FLAG_DEBUG 1u         /* keep per-step intermediates for pbf_get_grid / _neighbors / _fields */
+#define PBF_FLAG_DENSITY_STATS 2u /* compute density-error statistics every step */
+#define PBF_FLAG_GRAPH 4u         /* replay each step from a captured hipGraph */
+
+/* Solver parameters.  The first 13 fields are the reference's SolverParameter
+ * (include/solverBase.h:16-40) in the same order, size and meaning.  restRho is the reference's
+ * mass-free rest density (rho = sum of W, kernels/cudaPBFSolver.cu:170), i.e. rest density
+ * divided by particle mass.  The fields after `radius` are constants the reference hard-codes:
+ * gravity (cu:9: (0,-49,0)), the clamp jitter coefficient (solverBase.h:5: 0.01), the
+ * spiky-gradient cut-off below which the gradient is zero (cuh:107: 0.01), the grid
+ * dimensions (solverBase.h:11-13: 64^3) and the neighbour cap (solverBase.h:6: 63). */
+typedef struct pbf_params {
+    float h;
+    int32_t nParticles;
+    float restRho;
+    float epsilon;
+    float k;
+    float n;
+    float deltaq;
+    float c;
+    float dt;
+    float invDt;
+    int32_t iterations;
+    float vorticityConfinementEpilon;
+    float radius;
+    float gravity[3];
+    float jitter_coeff;
+    float grad_rmin;       /* spiky gradient is 0 below this |r| (cuh:107: 0.01) */
+    int32_t grid_dims[3];
+    int32_t max_neighbors; /* 1..255 */
+    uint32_t flags;        /* PBF_FLAG_* */
+} pbf_params;
+
+typedef struct pbf_solver pbf_solver;
+
+/* Replaces initCUDASolverDataBuffer (kernels/cudaPBF.h:8-10, kernels/cudaPBF.cu:4-30):
+ * allocates every device buffer for n particles and uploads positions and velocities.
+ * grid_space = {pmin xyz, pmax xyz}: only pmin anchors the grid (cuh:156-160), as in the
+ * reference where the app passes the box (src/renderapp.cpp:251).  boundary = the clamp box
+ * (box shrunk by the particle radius, renderapp.cpp:131).  vel_xyz may be NULL (zero). */
+int pbf_create(const pbf_params* para, const float grid_space[6], const float boundary[6], const float* pos_xyz,
+               const float* vel_xyz, int32_t n, pbf_solver** out);
+
+/* Replaces cudaPBFSolve (kernels/cudaPBF.h:12, kernels/cudaPBF.cu:50-114): one solver step
+ * with para's values (dt, iterations, ...).  para->nParticles and para->grid_dims must equal
+ * the solver's. */
+int pbf_solve(pbf_solver* s, const pbf_params* para);
+
+/* One step with the solver's current parameters and time step dt (para.dt = dt,
+ * para.invDt = 1/dt), as cudaPBFSolve called with an updated para.dt. */
+int pbf_step(pbf_solver* s, float dt);
+
+/* nsteps steps with the current parameters. */
+int pbf_advance(pbf_solver* s, int32_t nsteps);
+
+/* Host copy of the state in the caller's particle order (either pointer may be NULL). */
+int pbf_get_state(pbf_solver* s, float* pos_xyz, float* vel_xyz);
+
+/* Replace the state (caller's particle order; vel_xyz NULL = zero). */
+int pbf_set_state(pbf_solver* s, const float* pos_xyz, const float* vel_xyz);
+
+/* Write positions (and velocities if non-NULL) in the caller's particle order into DEVICE
+ * buffers the caller owns, on the solver's stream -- the reference's model where the position
+ * buffer belongs to the app (a mapped GL VBO, src/renderapp.cpp:236-249, cudaPBF.cu:7). */
+int pbf_export_device(pbf_solver* s, float* dev_pos_xyz, float* dev_vel_xyz);
+
+/* Replaces freeCUDASolverDataBuffer (kernels/cudaPBF.h:11, kernels/cudaPBF.cu:32-48). */
+int pbf_destroy(pbf_solver* s);
+
+/* Text of the calling thread's last error. */
+const char* pbf_last_error(void);
+
+/* Current parameters of the solver. */
+int pbf_get_params(pbf_solver* s, pbf_params* out);
+
+/* Block until the solver's queued work is done. */
+int pbf_synchronize(pbf_solver* s);
+
+/* Device view of the cell-sorted state after the last step: float4 (x,y,z,lambda) positions,
+ * float4 velocities, uint32 particle ids per slot. */
+typedef struct pbf_device_view {
+    const void* pos4;
+    const void* vel4;
+    const uint32_t* ids;
+    int32_t n;
+    int32_t ncells;
+} pbf_device_view;
+int pbf_device_state(pbf_solver* s, pbf_device_view* out);
+
+/* ---- diagnostics of the last step (need PBF_FLAG_DEBUG at create / solve time) ----------- */
+
+/* gid[n] by particle id (-1 = outside the grid), count[ncells], cell_start[ncells+1]
+ * (cell_start[ncells] = particles inside the grid), sorted_ids[n] (particle ids in cell
+ * order, inside-grid particles first).  Any pointer may be NULL.  Returns the number of
+ * particles inside the grid (>= 0) or an error. */
+int pbf_get_grid(pbf_solver* s, int32_t* gid, int32_t* count, int32_t* cell_start, int32_t* sorted_ids);
+
+/* Frozen neighbour lists by particle id: nbr[n*max_neighbors] (particle ids, list order),
+ * cnt[n]. */
+int pbf_get_neighbors(pbf_solver* s, int32_t* nbr, int32_t* cnt);
+
+/* lambda of the first and last Jacobi iteration, density error rho/restRho-1 at the end of the
+ * constraint loop, curl (xyz) -- all by particle id; any pointer may be NULL. */
+int pbf_get_fields(pbf_solver* s, float* lam_first, float* lam_last, float* err, float* curl_xyz);
+
+/* Density-error statistics of the last step: out[0] = mean |err|, out[1] = mean max(err,0),
+ * out[2] = max err.  Needs PBF_FLAG_DENSITY_STATS or PBF_FLAG_DEBUG. */
+int pbf_density_stats(pbf_solver* s, float out[3]);
+
+/* ---- measurement -------------------------------------------------------------------------- */
+
+/* Runs nsteps steps and returns the elapsed time measured with HIP events recorded on the
+ * solver's stream around them. */
+int pbf_time_steps(pbf_solver* s, int32_t nsteps, float* ms_total);
+
+/* Per-stage times of one step (HIP events between stages, one step, no graph):
+ * out[0] predict+hash, [1] scan, [2] scatter+rank+reorder, [3] neighbours+lambda1,
+ * [4] Jacobi iterations, [5] curl, [6] vorticity+viscosity.  ms. */
+int pbf_stage_times(pbf_solver* s, float out[7]);
+
+/* One step launched eagerly with a HIP event recorded on the solver's stream after every
+ * launch group; ms[i] is the time of group i and names[i] its name (static strings:
+ * predict_hash, scan, scatter_rank_reorder, neighbors_lambda, lambda, delta_apply, curl,
+ * vort_apply).  Returns the number of groups written (<= max) or an error. */
+int pbf_profile_step(pbf_solver* s, float* ms, const char** names, int32_t max);
+
+/* Library build string. */
+const char* pbf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PBF_H_ */

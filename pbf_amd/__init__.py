@@ -1,0 +1,384 @@
+"""pbf_amd -- Position Based Fluids on MI355X (gfx950), host side.
+
+Python mirror of the reference's solver host API (kernels/cudaPBF.h:8-12) over the C ABI of
+``libpbf_hip.so`` (include/pbf.h):
+
+=====================================  ==========================================
+reference                              here
+=====================================  ==========================================
+``SolverParameter`` solverBase.h:16    :class:`SolverParameter` (same 13 leading fields)
+``SolverData`` solverBase.h:45         :class:`SolverData` (owns one ``pbf_solver``)
+``initCUDASolverDataBuffer`` cu:4      :func:`initCUDASolverDataBuffer`
+``cudaPBFSolve`` cu:50                 :func:`cudaPBFSolve`
+``freeCUDASolverDataBuffer`` cu:32     :func:`freeCUDASolverDataBuffer`
+``PBFSolver`` solver.h:83              :class:`PBFSolver` (``solve(particles)`` in place)
+=====================================  ==========================================
+
+:class:`Solver` is the direct handle API used by the tests and the bench.  Every call goes to
+the HIP library; there is no CPU fallback, and importing this package does not need a GPU,
+but creating a solver does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpbf_hip.so")
+
+PBF_OK = 0
+PBF_ERR_ARG = -1
+PBF_ERR_HIP = -2
+PBF_ERR_CAPACITY = -3
+PBF_ERR_STATE = -4
+PBF_ERR_NOGPU = -5
+
+FLAG_DEBUG = 1
+FLAG_DENSITY_STATS = 2
+FLAG_GRAPH = 4
+
+
+class PBFError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"pbf error {code}: {msg}")
+        self.code = code
+
+
+class SolverParameter(C.Structure):
+    """``pbf_params``: the reference's SolverParameter (include/solverBase.h:16-40) followed by
+    the constants the reference hard-codes (gravity, jitter coefficient, grid dims, neighbour
+    cap) and flags."""
+
+    _fields_ = [
+        ("h", C.c_float),
+        ("nParticles", C.c_int32),
+        ("restRho", C.c_float),
+        ("epsilon", C.c_float),
+        ("k", C.c_float),
+        ("n", C.c_float),
+        ("deltaq", C.c_float),
+        ("c", C.c_float),
+        ("dt", C.c_float),
+        ("invDt", C.c_float),
+        ("iterations", C.c_int32),
+        ("vorticityConfinementEpilon", C.c_float),
+        ("radius", C.c_float),
+        ("gravity", C.c_float * 3),
+        ("jitter_coeff", C.c_float),
+        ("grad_rmin", C.c_float),
+        ("grid_dims", C.c_int32 * 3),
+        ("max_neighbors", C.c_int32),
+        ("flags", C.c_uint32),
+    ]
+
+    @classmethod
+    def from_dict(cls, p: dict, n: int, grid_dims, flags: int = 0) -> "SolverParameter":
+        s = cls()
+        s.h = p["h"]
+        s.nParticles = n
+        s.restRho = p["restRho"]
+        s.epsilon = p["epsilon"]
+        s.k = p["k"]
+        s.n = p["n"]
+        s.deltaq = p["deltaq"]
+        s.c = p["c"]
+        s.dt = np.float32(p["dt"])
+        s.invDt = np.float32(1.0) / np.float32(p["dt"])
+        s.iterations = int(p["iterations"])
+        s.vorticityConfinementEpilon = p["vorticityConfinementEpilon"]
+        s.radius = p["radius"]
+        for a in range(3):
+            s.gravity[a] = p["gravity"][a]
+            s.grid_dims[a] = int(grid_dims[a])
+        s.jitter_coeff = p["jitter_coeff"]
+        s.grad_rmin = p.get("grad_rmin", 0.01)
+        s.max_neighbors = int(p.get("max_neighbors", 63))
+        s.flags = int(flags)
+        return s
+
+    def copy(self) -> "SolverParameter":
+        o = SolverParameter()
+        C.memmove(C.byref(o), C.byref(self), C.sizeof(self))
+        return o
+
+
+class DeviceView(C.Structure):
+    _fields_ = [
+        ("pos4", C.c_void_p),
+        ("vel4", C.c_void_p),
+        ("ids", C.c_void_p),
+        ("n", C.c_int32),
+        ("ncells", C.c_int32),
+    ]
+
+
+_lib_handle = None
+
+_F = C.POINTER(C.c_float)
+_I = C.POINTER(C.c_int32)
+_VP = C.c_void_p
+
+
+def lib() -> C.CDLL:
+    """The loaded HIP library; raises if it has not been built."""
+    global _lib_handle
+    if _lib_handle is not None:
+        return _lib_handle
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m pbf_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "pbf_create": (C.c_int, [C.POINTER(SolverParameter), _F, _F, _F, _F, C.c_int32, C.POINTER(_VP)]),
+        "pbf_solve": (C.c_int, [_VP, C.POINTER(SolverParameter)]),
+        "pbf_step": (C.c_int, [_VP, C.c_float]),
+        "pbf_advance": (C.c_int, [_VP, C.c_int32]),
+        "pbf_get_state": (C.c_int, [_VP, _F, _F]),
+        "pbf_set_state": (C.c_int, [_VP, _F, _F]),
+        "pbf_export_device": (C.c_int, [_VP, _VP, _VP]),
+        "pbf_destroy": (C.c_int, [_VP]),
+        "pbf_last_error": (C.c_char_p, []),
+        "pbf_get_params": (C.c_int, [_VP, C.POINTER(SolverParameter)]),
+        "pbf_synchronize": (C.c_int, [_VP]),
+        "pbf_device_state": (C.c_int, [_VP, C.POINTER(DeviceView)]),
+        "pbf_get_grid": (C.c_int, [_VP, _I, _I, _I, _I]),
+        "pbf_get_neighbors": (C.c_int, [_VP, _I, _I]),
+        "pbf_get_fields": (C.c_int, [_VP, _F, _F, _F, _F]),
+        "pbf_density_stats": (C.c_int, [_VP, _F]),
+        "pbf_time_steps": (C.c_int, [_VP, C.c_int32, _F]),
+        "pbf_stage_times": (C.c_int, [_VP, _F]),
+        "pbf_version": (C.c_char_p, []),
+        "pbf_profile_step": (C.c_int, [_VP, _F, C.POINTER(C.c_char_p), C.c_int32]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib_handle = L
+    return L
+
+
+def _check(rc: int) -> int:
+    if rc < 0:
+        raise PBFError(rc, lib().pbf_last_error().decode())
+    return rc
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(_F)
+
+
+def _ip(a):
+    return None if a is None else a.ctypes.data_as(_I)
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+class Solver:
+    """One GPU solver (a ``pbf_solver`` handle)."""
+
+    def __init__(self, para: SolverParameter, grid_space, boundary, pos, vel=None):
+        pos = _f32(pos, (-1, 3))
+        n = pos.shape[0]
+        vel = None if vel is None else _f32(vel, (n, 3))
+        gs = _f32(grid_space, (6,))
+        bd = _f32(boundary, (6,))
+        self.para = para.copy()
+        self.para.nParticles = n
+        self.n = n
+        self.ncells = int(para.grid_dims[0] * para.grid_dims[1] * para.grid_dims[2])
+        h = _VP()
+        _check(lib().pbf_create(C.byref(self.para), _fp(gs), _fp(bd), _fp(pos), _fp(vel), n, C.byref(h)))
+        self._h = h
+
+    @classmethod
+    def from_scene(cls, scene, flags: int = 0, **over) -> "Solver":
+        p = dict(scene.params)
+        p.update(over)
+        para = SolverParameter.from_dict(p, scene.n, scene.grid_dims, flags)
+        return cls(para, scene.box, scene.boundary, scene.pos, scene.vel)
+
+    # lifecycle --------------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pbf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # stepping ---------------------------------------------------------------------------
+    def solve(self, para: SolverParameter | None = None):
+        """One step with ``para`` (the solver's own when None), as cudaPBFSolve."""
+        if para is not None:
+            self.para = para.copy()
+        _check(lib().pbf_solve(self._h, C.byref(self.para)))
+
+    def step(self, dt: float | None = None):
+        _check(lib().pbf_step(self._h, float(self.para.dt if dt is None else dt)))
+
+    def advance(self, nsteps: int):
+        _check(lib().pbf_advance(self._h, int(nsteps)))
+
+    def synchronize(self):
+        _check(lib().pbf_synchronize(self._h))
+
+    # state ------------------------------------------------------------------------------
+    def get_state(self):
+        pos = np.empty((self.n, 3), np.float32)
+        vel = np.empty((self.n, 3), np.float32)
+        _check(lib().pbf_get_state(self._h, _fp(pos), _fp(vel)))
+        return pos, vel
+
+    def set_state(self, pos, vel=None):
+        pos = _f32(pos, (self.n, 3))
+        vel = None if vel is None else _f32(vel, (self.n, 3))
+        _check(lib().pbf_set_state(self._h, _fp(pos), _fp(vel)))
+
+    def export_device(self, dev_pos: int, dev_vel: int | None = None):
+        _check(lib().pbf_export_device(self._h, dev_pos, dev_vel))
+
+    def device_view(self) -> DeviceView:
+        v = DeviceView()
+        _check(lib().pbf_device_state(self._h, C.byref(v)))
+        return v
+
+    # diagnostics ------------------------------------------------------------------------
+    def grid(self):
+        """(gid[n], count[ncells], cell_start[ncells+1], sorted_ids[n], n_in_grid) of the last step."""
+        gid = np.empty(self.n, np.int32)
+        cnt = np.empty(self.ncells, np.int32)
+        cs = np.empty(self.ncells + 1, np.int32)
+        srt = np.empty(self.n, np.int32)
+        nin = _check(lib().pbf_get_grid(self._h, _ip(gid), _ip(cnt), _ip(cs), _ip(srt)))
+        return gid, cnt, cs, srt, nin
+
+    def neighbors(self):
+        cap = int(self.para.max_neighbors)
+        nbr = np.empty((self.n, cap), np.int32)
+        cnt = np.empty(self.n, np.int32)
+        _check(lib().pbf_get_neighbors(self._h, _ip(nbr), _ip(cnt)))
+        return nbr, cnt
+
+    def fields(self):
+        """(lambda_first, lambda_last, density_error, curl) of the last step, by particle id."""
+        lf = np.empty(self.n, np.float32)
+        ll = np.empty(self.n, np.float32)
+        er = np.empty(self.n, np.float32)
+        cu = np.empty((self.n, 3), np.float32)
+        _check(lib().pbf_get_fields(self._h, _fp(lf), _fp(ll), _fp(er), _fp(cu)))
+        return lf, ll, er, cu
+
+    def density_stats(self):
+        out = np.empty(3, np.float32)
+        _check(lib().pbf_density_stats(self._h, _fp(out)))
+        return float(out[0]), float(out[1]), float(out[2])
+
+    # measurement ------------------------------------------------------------------------
+    def time_steps(self, nsteps: int) -> float:
+        ms = np.zeros(1, np.float32)
+        _check(lib().pbf_time_steps(self._h, int(nsteps), _fp(ms)))
+        return float(ms[0])
+
+    def profile_step(self):
+        """[(stage name, ms)] of one eagerly launched step, HIP events between launch groups."""
+        ms = np.zeros(64, np.float32)
+        names = (C.c_char_p * 64)()
+        cnt = _check(lib().pbf_profile_step(self._h, _fp(ms), names, 64))
+        return [(names[i].decode(), float(ms[i])) for i in range(cnt)]
+
+    def stage_times(self):
+        out = np.zeros(7, np.float32)
+        _check(lib().pbf_stage_times(self._h, _fp(out)))
+        return out
+
+
+# ---- the reference's host API names (kernels/cudaPBF.h:8-12) ------------------------------
+class SolverData:
+    """Holds the device solver that initCUDASolverDataBuffer creates (solverBase.h:45-64)."""
+
+    def __init__(self):
+        self.solver: Solver | None = None
+
+
+def initCUDASolverDataBuffer(position, velocity, data: SolverData, para: SolverParameter, gridSpace, boundary):
+    """kernels/cudaPBF.cu:4-30: allocate the device state and upload x, v."""
+    pos = _f32(position, (-1, 3))
+    if int(para.nParticles) != pos.shape[0]:
+        raise PBFError(PBF_ERR_ARG, "para.nParticles differs from len(position)")
+    data.solver = Solver(para, gridSpace, boundary, pos, velocity)
+
+
+def cudaPBFSolve(data: SolverData, para: SolverParameter):
+    """kernels/cudaPBF.cu:50-114: one step with para (dt, iterations, ...)."""
+    if data.solver is None:
+        raise PBFError(PBF_ERR_STATE, "solver data not initialised")
+    data.solver.solve(para)
+
+
+def freeCUDASolverDataBuffer(data: SolverData):
+    """kernels/cudaPBF.cu:32-48."""
+    if data.solver is not None:
+        data.solver.close()
+        data.solver = None
+
+
+init_solver_data_buffer = initCUDASolverDataBuffer
+pbf_solve = cudaPBFSolve
+free_solver_data_buffer = freeCUDASolverDataBuffer
+
+
+class PBFSolver:
+    """The reference's CPU-side solver shape (include/solver.h:83-111) backed by the GPU solver.
+
+    ``PBFSolver(para, box)`` -- the clamp boundary is ``box`` shrunk by ``para.radius``
+    (solver.h:97) and the grid is anchored at ``box.pMin`` (solver.h:109).  ``solve(particles)``
+    advances one step in place on a structured array with ``position`` and ``velocity``
+    fields (include/particle.h:9-15).  ``fext`` is accepted as in solver.h:86-88; like the
+    reference (solver.h:201) the step does not call it.
+    """
+
+    PARTICLE_DTYPE = np.dtype([("position", np.float32, 3), ("velocity", np.float32, 3)])
+
+    def __init__(self, para: SolverParameter, box, fext=None):
+        self.para = para.copy()
+        self.box = _f32(box, (6,))
+        r = np.float32(para.radius)
+        self.boundary = np.concatenate([self.box[:3] + r, self.box[3:] - r]).astype(np.float32)
+        self.fext = fext
+        self._solver: Solver | None = None
+
+    def solve(self, particles: np.ndarray):
+        pos = np.ascontiguousarray(particles["position"], np.float32)
+        vel = np.ascontiguousarray(particles["velocity"], np.float32)
+        if self._solver is None or self._solver.n != pos.shape[0]:
+            if self._solver is not None:
+                self._solver.close()
+            self.para.nParticles = pos.shape[0]
+            self._solver = Solver(self.para, self.box, self.boundary, pos, vel)
+        else:
+            self._solver.set_state(pos, vel)
+        self._solver.solve(self.para)
+        p, v = self._solver.get_state()
+        particles["position"] = p
+        particles["velocity"] = v
+
+    def close(self):
+        if self._solver is not None:
+            self._solver.close()
+            self._solver = None

@@ -1,0 +1,608 @@
+// Host runtime and C ABI (include/pbf.h) of the gfx950 PBF solver.
+//
+// One pbf_solver owns its device buffers and one HIP stream.  A step is the launch sequence
+// of the reference's cudaPBFSolve (kernels/cudaPBF.cu:50-114) without its per-launch device
+// synchronisations: predict+hash+count, scan, scatter, rank+reorder, neighbours+lambda,
+// I x (delta-p+apply [, lambda]), curl, vorticity+viscosity+apply.  Optionally the whole step
+// is captured once into a hipGraph and replayed.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pbf.h"
+#include "pbf_kernels.h"
+
+using namespace pbf;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                     \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess) return set_error(PBF_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                               __FILE__, __LINE__);                                       \
+    } while (0)
+
+const double kInvPi = 0.31830988618;  // INVPI, include/solverBase.h:4
+
+}  // namespace
+
+struct pbf_solver {
+    pbf_params P;
+    float gs[6];
+    float bd[6];
+    int n = 0;
+    int ncells = 0;
+    int nstride = 0;
+    int cap = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    StepBuffers B{};
+    int pc = 0, vc = 0, ic = 0;  // which of pos/vel/ids[2] hold the state
+    int lam_last = -1;           // pos buffer whose .w holds the last iteration's lambda
+    int steps_done = 0;
+    // diagnostics
+    int32_t* dbg_gid = nullptr;
+    float* dbg_lam = nullptr;
+    float* dbg_err = nullptr;
+    float* partial = nullptr;
+    int npartial = 0;
+    bool stats_valid = false;
+    // per-launch profiling: when set, enqueue_step records an event after every launch
+    std::vector<hipEvent_t>* prof_events = nullptr;
+    std::vector<const char*>* prof_names = nullptr;
+    // graph replay, one graph per state parity
+    hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+StepConsts make_consts(const pbf_solver* s) {
+    const pbf_params& P = s->P;
+    StepConsts K;
+    std::memset(&K, 0, sizeof(K));
+    K.h = P.h;
+    K.h2 = P.h * P.h;
+    float h9 = (float)std::pow((double)P.h, 9.0);
+    float h6 = (float)std::pow((double)P.h, 6.0);
+    K.c6 = (float)((315.0 * kInvPi) / (double)(64.0f * h9));
+    K.cs = (float)((-45.0 * kInvPi) / (double)h6);
+    float dq2 = P.deltaq * P.deltaq + 0.0f * 0.0f + 0.0f * 0.0f;
+    if (dq2 > K.h2) {
+        K.wdq = 0.0f;
+    } else {
+        float d = K.h2 - dq2;
+        K.wdq = K.c6 * (d * d * d);
+    }
+    K.k = P.k;
+    K.scorr_n = P.n;
+    K.n_int = (P.n >= 0.0f && P.n <= 64.0f && P.n == (float)(int)P.n) ? (int)P.n : -1;
+    K.restRho = P.restRho;
+    K.eps = P.epsilon;
+    K.dt = P.dt;
+    K.visc_c = P.c;
+    K.vort_eps = P.vorticityConfinementEpilon;
+    K.jitter = P.jitter_coeff;
+    K.rmin = P.grad_rmin;
+    for (int a = 0; a < 3; ++a) {
+        K.g[a] = P.gravity[a];
+        K.gmin[a] = s->gs[a];
+        K.dims[a] = P.grid_dims[a];
+        K.bmin[a] = s->bd[a];
+        K.bmax[a] = s->bd[3 + a];
+    }
+    K.invH = 1.0f / P.h;
+    K.ncells = s->ncells;
+    K.cap = s->cap;
+    K.n = s->n;
+    K.nstride = s->nstride;
+    return K;
+}
+
+int check_params(const pbf_params* p) {
+    if (!p) return set_error(PBF_ERR_ARG, "params is NULL");
+    if (!(p->h > 0.0f)) return set_error(PBF_ERR_ARG, "h must be > 0");
+    if (p->iterations < 0 || p->iterations > 1000) return set_error(PBF_ERR_ARG, "iterations out of range");
+    if (p->max_neighbors < 1 || p->max_neighbors > 255)
+        return set_error(PBF_ERR_ARG, "max_neighbors must be in 1..255");
+    for (int a = 0; a < 3; ++a)
+        if (p->grid_dims[a] < 1 || p->grid_dims[a] > 4096) return set_error(PBF_ERR_ARG, "grid_dims out of range");
+    int64_t c = (int64_t)p->grid_dims[0] * p->grid_dims[1] * p->grid_dims[2];
+    if (c > (int64_t)1 << 28) return set_error(PBF_ERR_ARG, "too many grid cells");
+    return PBF_OK;
+}
+
+template <class T>
+int dalloc(pbf_solver* s, T** p, size_t count) {
+    size_t bytes = count * sizeof(T);
+    if (bytes == 0) bytes = 256;
+    void* q = nullptr;
+    HIP_TRY(hipMalloc(&q, bytes));
+    s->allocs.push_back(q);
+    *p = (T*)q;
+    return PBF_OK;
+}
+
+void free_all(pbf_solver* s) {
+    for (int i = 0; i < 4; ++i)
+        if (s->graph[i]) {
+            (void)hipGraphExecDestroy(s->graph[i]);
+            s->graph[i] = nullptr;
+        }
+    for (void* p : s->allocs) (void)hipFree(p);
+    s->allocs.clear();
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    s->stream = nullptr;
+}
+
+void drop_graphs(pbf_solver* s) {
+    for (int i = 0; i < 4; ++i)
+        if (s->graph[i]) {
+            (void)hipGraphExecDestroy(s->graph[i]);
+            s->graph[i] = nullptr;
+        }
+}
+
+// Enqueue one step.  Stage events (7 stage boundaries + start) are recorded when ev != NULL.
+int prof_mark(pbf_solver* s, const char* name) {
+    if (!s->prof_events) return PBF_OK;
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    HIP_TRY(hipEventRecord(e, s->stream));
+    s->prof_events->push_back(e);
+    s->prof_names->push_back(name);
+    return PBF_OK;
+}
+
+#define PROF(name)                          \
+    do {                                    \
+        int rc_ = prof_mark(s, name);       \
+        if (rc_) return rc_;                \
+    } while (0)
+
+int enqueue_step(pbf_solver* s, hipEvent_t* ev) {
+    const StepConsts K = make_consts(s);
+    StepBuffers& B = s->B;
+    hipStream_t st = s->stream;
+    const bool dbg = (s->P.flags & PBF_FLAG_DEBUG) != 0;
+    const bool stats = dbg || (s->P.flags & PBF_FLAG_DENSITY_STATS);
+    if (ev) HIP_TRY(hipEventRecord(ev[0], st));
+    PROF("start");
+    launch_predict_hash(st, K, B.pos[s->pc], B.vel[s->vc], B);
+    PROF("predict_hash");
+    if (ev) HIP_TRY(hipEventRecord(ev[1], st));
+    launch_scan(st, K, B);
+    PROF("scan");
+    if (ev) HIP_TRY(hipEventRecord(ev[2], st));
+    launch_sort_reorder(st, K, B, B.pos[s->pc], B.ids[s->ic], B.pos[s->pc ^ 1], B.vel[s->vc ^ 1], B.ids[s->ic ^ 1],
+                        dbg ? s->dbg_gid : nullptr);
+    PROF("scatter_rank_reorder");
+    if (ev) HIP_TRY(hipEventRecord(ev[3], st));
+    int a = s->pc ^ 1;
+    int v = s->vc ^ 1;
+    launch_neighbors_lambda(st, K, B.pos[a], B, dbg ? s->dbg_lam : nullptr);
+    PROF("neighbors_lambda");
+    if (ev) HIP_TRY(hipEventRecord(ev[4], st));
+    const int I = s->P.iterations;
+    for (int it = 0; it < I; ++it) {
+        if (it > 0) {
+            launch_lambda(st, K, B.pos[a], B);
+            PROF("lambda");
+        }
+        launch_delta_apply(st, K, B.pos[a], B.pos[a ^ 1], B.vel[v], B, it == I - 1);
+        PROF("delta_apply");
+        a ^= 1;
+    }
+    if (ev) HIP_TRY(hipEventRecord(ev[5], st));
+    launch_curl(st, K, B.pos[a], B.vel[v], B, dbg ? s->dbg_err : nullptr, stats ? s->partial : nullptr);
+    PROF("curl");
+    if (ev) HIP_TRY(hipEventRecord(ev[6], st));
+    launch_vort_apply(st, K, B.pos[a], B.vel[v], B.vel[v ^ 1], B);
+    PROF("vort_apply");
+    if (ev) HIP_TRY(hipEventRecord(ev[7], st));
+    HIP_TRY(hipGetLastError());
+    s->lam_last = I > 0 ? (a ^ 1) : -1;
+    s->pc = a;
+    s->vc = v ^ 1;
+    s->ic ^= 1;
+    s->stats_valid = stats;
+    s->steps_done++;
+    return PBF_OK;
+}
+
+int run_step(pbf_solver* s) {
+    HIP_TRY(hipSetDevice(s->device));
+    if (!(s->P.flags & PBF_FLAG_GRAPH) || (s->P.flags & PBF_FLAG_DEBUG)) return enqueue_step(s, nullptr);
+    // A captured step bakes in which buffer of each pair it reads; vc is unchanged by a step,
+    // so (pc, ic) names the variant.
+    int key = s->pc * 2 + s->ic;
+    if (!s->graph[key]) {
+        int pc0 = s->pc, vc0 = s->vc, ic0 = s->ic, ll0 = s->lam_last, sd0 = s->steps_done;
+        HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_step(s, nullptr);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s->stream, &g);
+        if (rc != PBF_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return set_error(PBF_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+        hipGraphExec_t ge = nullptr;
+        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return set_error(PBF_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+        s->graph[key] = ge;
+        // capture only recorded the step; restore the parity and launch it for real below
+        s->pc = pc0;
+        s->vc = vc0;
+        s->ic = ic0;
+        s->lam_last = ll0;
+        s->steps_done = sd0;
+    }
+    HIP_TRY(hipGraphLaunch(s->graph[key], s->stream));
+    // advance the parity exactly as enqueue_step does
+    int a = s->pc ^ 1;
+    for (int it = 0; it < s->P.iterations; ++it) a ^= 1;
+    s->lam_last = s->P.iterations > 0 ? (a ^ 1) : -1;
+    s->pc = a;
+    s->ic ^= 1;
+    s->stats_valid = (s->P.flags & PBF_FLAG_DENSITY_STATS) != 0;
+    s->steps_done++;
+    return PBF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pbf_last_error(void) { return g_last_error.c_str(); }
+
+const char* pbf_version(void) { return "pbf_hip 0.1 gfx950"; }
+
+int pbf_create(const pbf_params* para, const float grid_space[6], const float boundary[6], const float* pos_xyz,
+               const float* vel_xyz, int32_t n, pbf_solver** out) {
+    if (!out) return set_error(PBF_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int rc = check_params(para);
+    if (rc) return rc;
+    if (n < 0) return set_error(PBF_ERR_ARG, "n < 0");
+    if (n > 0 && !pos_xyz) return set_error(PBF_ERR_ARG, "pos_xyz is NULL");
+    if (!grid_space || !boundary) return set_error(PBF_ERR_ARG, "grid_space / boundary is NULL");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(PBF_ERR_NOGPU, "no HIP device");
+    pbf_solver* s = new pbf_solver();
+    s->P = *para;
+    s->P.nParticles = n;
+    std::memcpy(s->gs, grid_space, sizeof(s->gs));
+    std::memcpy(s->bd, boundary, sizeof(s->bd));
+    s->n = n;
+    s->ncells = para->grid_dims[0] * para->grid_dims[1] * para->grid_dims[2];
+    s->nstride = ((n + 63) / 64) * 64;
+    if (s->nstride == 0) s->nstride = 64;
+    s->cap = para->max_neighbors;
+    (void)hipGetDevice(&s->device);
+    auto fail = [&](int code) {
+        free_all(s);
+        delete s;
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_error(PBF_ERR_HIP, "hipStreamCreate failed"));
+    StepBuffers& B = s->B;
+    size_t np = (size_t)std::max(n, 1);
+    int m = s->ncells + 1;
+    size_t mpad = (size_t)scan_tiles(m) * 1024;
+    for (int i = 0; i < 2; ++i) {
+        if ((rc = dalloc(s, &B.pos[i], np))) return fail(rc);
+        if ((rc = dalloc(s, &B.vel[i], np))) return fail(rc);
+        if ((rc = dalloc(s, &B.ids[i], np))) return fail(rc);
+    }
+    if ((rc = dalloc(s, &B.xold, np)) || (rc = dalloc(s, &B.xp, np)) || (rc = dalloc(s, &B.vp, np)) ||
+        (rc = dalloc(s, &B.omega, np)) || (rc = dalloc(s, &B.cell, np)) || (rc = dalloc(s, &B.local, np)) ||
+        (rc = dalloc(s, &B.count, mpad)) || (rc = dalloc(s, &B.cstart, mpad + 4)) ||
+        (rc = dalloc(s, &B.tile_sum, (size_t)scan_tiles(m) + 4)) || (rc = dalloc(s, &B.bucket, np)) ||
+        (rc = dalloc(s, &B.nbr, (size_t)s->cap * s->nstride)) || (rc = dalloc(s, &B.ncnt, np + 16)))
+        return fail(rc);
+    s->npartial = err_partial_blocks((int)np);
+    if ((rc = dalloc(s, &s->dbg_gid, np)) || (rc = dalloc(s, &s->dbg_lam, np)) || (rc = dalloc(s, &s->dbg_err, np)) ||
+        (rc = dalloc(s, &s->partial, (size_t)s->npartial * 3)))
+        return fail(rc);
+    launch_fill_u32(s->stream, B.count, mpad, 0u);
+    if (hipGetLastError() != hipSuccess) return fail(set_error(PBF_ERR_HIP, "fill launch failed"));
+    *out = s;
+    rc = pbf_set_state(s, pos_xyz, vel_xyz);
+    if (rc) {
+        *out = nullptr;
+        return fail(rc);
+    }
+    return PBF_OK;
+}
+
+int pbf_set_state(pbf_solver* s, const float* pos_xyz, const float* vel_xyz) {
+    if (!s) return set_error(PBF_ERR_ARG, "solver is NULL");
+    if (s->n == 0) return PBF_OK;
+    if (!pos_xyz) return set_error(PBF_ERR_ARG, "pos_xyz is NULL");
+    HIP_TRY(hipSetDevice(s->device));
+    size_t bytes = (size_t)s->n * 3 * sizeof(float);
+    // stage through the (not yet used) xp / vp buffers
+    float* dp = reinterpret_cast<float*>(s->B.xp);
+    float* dv = reinterpret_cast<float*>(s->B.vp);
+    HIP_TRY(hipMemcpyAsync(dp, pos_xyz, bytes, hipMemcpyHostToDevice, s->stream));
+    if (vel_xyz) {
+        HIP_TRY(hipMemcpyAsync(dv, vel_xyz, bytes, hipMemcpyHostToDevice, s->stream));
+    } else {
+        launch_fill_u32(s->stream, reinterpret_cast<uint32_t*>(dv), (size_t)s->n * 3, 0u);
+    }
+    s->pc = s->vc = s->ic = 0;
+    launch_import(s->stream, s->n, dp, dv, s->B.pos[0], s->B.vel[0], s->B.ids[0]);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    drop_graphs(s);
+    s->lam_last = -1;
+    s->stats_valid = false;
+    return PBF_OK;
+}
+
+int pbf_destroy(pbf_solver* s) {
+    if (!s) return PBF_OK;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    free_all(s);
+    delete s;
+    return PBF_OK;
+}
+
+int pbf_get_params(pbf_solver* s, pbf_params* out) {
+    if (!s || !out) return set_error(PBF_ERR_ARG, "NULL argument");
+    *out = s->P;
+    return PBF_OK;
+}
+
+int pbf_solve(pbf_solver* s, const pbf_params* para) {
+    if (!s) return set_error(PBF_ERR_ARG, "solver is NULL");
+    int rc = check_params(para);
+    if (rc) return rc;
+    if (para->nParticles != s->n) return set_error(PBF_ERR_CAPACITY, "nParticles differs from the solver's");
+    if (para->grid_dims[0] != s->P.grid_dims[0] || para->grid_dims[1] != s->P.grid_dims[1] ||
+        para->grid_dims[2] != s->P.grid_dims[2])
+        return set_error(PBF_ERR_CAPACITY, "grid_dims differ from the solver's");
+    if (para->max_neighbors != s->cap) return set_error(PBF_ERR_CAPACITY, "max_neighbors differs from the solver's");
+    if (std::memcmp(para, &s->P, sizeof(pbf_params)) != 0) {
+        s->P = *para;
+        drop_graphs(s);
+    }
+    return run_step(s);
+}
+
+int pbf_step(pbf_solver* s, float dt) {
+    if (!s) return set_error(PBF_ERR_ARG, "solver is NULL");
+    if (!(dt > 0.0f)) return set_error(PBF_ERR_ARG, "dt must be > 0");
+    if (s->P.dt != dt) {
+        s->P.dt = dt;
+        s->P.invDt = 1.0f / dt;
+        drop_graphs(s);
+    }
+    return run_step(s);
+}
+
+int pbf_advance(pbf_solver* s, int32_t nsteps) {
+    if (!s) return set_error(PBF_ERR_ARG, "solver is NULL");
+    for (int i = 0; i < nsteps; ++i) {
+        int rc = run_step(s);
+        if (rc) return rc;
+    }
+    return PBF_OK;
+}
+
+int pbf_synchronize(pbf_solver* s) {
+    if (!s) return set_error(PBF_ERR_ARG, "solver is NULL");
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return PBF_OK;
+}
+
+int pbf_export_device(pbf_solver* s, float* dev_pos_xyz, float* dev_vel_xyz) {
+    if (!s) return set_error(PBF_ERR_ARG, "solver is NULL");
+    HIP_TRY(hipSetDevice(s->device));
+    launch_export(s->stream, s->n, s->B.pos[s->pc], s->B.vel[s->vc], s->B.ids[s->ic], dev_pos_xyz, dev_vel_xyz);
+    HIP_TRY(hipGetLastError());
+    return PBF_OK;
+}
+
+int pbf_get_state(pbf_solver* s, float* pos_xyz, float* vel_xyz) {
+    if (!s) return set_error(PBF_ERR_ARG, "solver is NULL");
+    if (s->n == 0) return PBF_OK;
+    HIP_TRY(hipSetDevice(s->device));
+    // xp / vp are free between steps: use them as staging
+    float* dp = reinterpret_cast<float*>(s->B.xp);
+    float* dv = reinterpret_cast<float*>(s->B.vp);
+    launch_export(s->stream, s->n, s->B.pos[s->pc], s->B.vel[s->vc], s->B.ids[s->ic], dp, dv);
+    HIP_TRY(hipGetLastError());
+    size_t bytes = (size_t)s->n * 3 * sizeof(float);
+    if (pos_xyz) HIP_TRY(hipMemcpyAsync(pos_xyz, dp, bytes, hipMemcpyDeviceToHost, s->stream));
+    if (vel_xyz) HIP_TRY(hipMemcpyAsync(vel_xyz, dv, bytes, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return PBF_OK;
+}
+
+int pbf_device_state(pbf_solver* s, pbf_device_view* out) {
+    if (!s || !out) return set_error(PBF_ERR_ARG, "NULL argument");
+    out->pos4 = s->B.pos[s->pc];
+    out->vel4 = s->B.vel[s->vc];
+    out->ids = s->B.ids[s->ic];
+    out->n = s->n;
+    out->ncells = s->ncells;
+    return PBF_OK;
+}
+
+int pbf_get_grid(pbf_solver* s, int32_t* gid, int32_t* count, int32_t* cell_start, int32_t* sorted_ids) {
+    if (!s) return set_error(PBF_ERR_ARG, "solver is NULL");
+    if (!(s->P.flags & PBF_FLAG_DEBUG) || s->steps_done == 0)
+        return set_error(PBF_ERR_STATE, "needs a step taken with PBF_FLAG_DEBUG");
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    std::vector<uint32_t> cs(s->ncells + 1);
+    HIP_TRY(hipMemcpy(cs.data(), s->B.cstart, sizeof(uint32_t) * cs.size(), hipMemcpyDeviceToHost));
+    if (gid && s->n) HIP_TRY(hipMemcpy(gid, s->dbg_gid, sizeof(int32_t) * s->n, hipMemcpyDeviceToHost));
+    if (cell_start)
+        for (int c = 0; c <= s->ncells; ++c) cell_start[c] = (int32_t)cs[c];
+    if (count)
+        for (int c = 0; c < s->ncells; ++c) count[c] = (int32_t)(cs[c + 1] - cs[c]);
+    if (sorted_ids && s->n)
+        HIP_TRY(hipMemcpy(sorted_ids, s->B.ids[s->ic], sizeof(int32_t) * s->n, hipMemcpyDeviceToHost));
+    return (int)cs[s->ncells];
+}
+
+int pbf_get_neighbors(pbf_solver* s, int32_t* nbr, int32_t* cnt) {
+    if (!s) return set_error(PBF_ERR_ARG, "solver is NULL");
+    if (!(s->P.flags & PBF_FLAG_DEBUG) || s->steps_done == 0)
+        return set_error(PBF_ERR_STATE, "needs a step taken with PBF_FLAG_DEBUG");
+    if (s->n == 0) return PBF_OK;
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    std::vector<uint32_t> tab((size_t)s->cap * s->nstride), ids(s->n);
+    std::vector<uint8_t> c(s->n);
+    HIP_TRY(hipMemcpy(tab.data(), s->B.nbr, sizeof(uint32_t) * tab.size(), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ids.data(), s->B.ids[s->ic], sizeof(uint32_t) * s->n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(c.data(), s->B.ncnt, s->n, hipMemcpyDeviceToHost));
+    for (int sl = 0; sl < s->n; ++sl) {
+        uint32_t id = ids[sl];
+        if (cnt) cnt[id] = c[sl];
+        if (nbr)
+            for (int j = 0; j < s->cap; ++j)
+                nbr[(size_t)id * s->cap + j] = j < c[sl] ? (int32_t)ids[tab[(size_t)j * s->nstride + sl]] : -1;
+    }
+    return PBF_OK;
+}
+
+int pbf_get_fields(pbf_solver* s, float* lam_first, float* lam_last, float* err, float* curl_xyz) {
+    if (!s) return set_error(PBF_ERR_ARG, "solver is NULL");
+    if (!(s->P.flags & PBF_FLAG_DEBUG) || s->steps_done == 0)
+        return set_error(PBF_ERR_STATE, "needs a step taken with PBF_FLAG_DEBUG");
+    if (s->n == 0) return PBF_OK;
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    int n = s->n;
+    std::vector<uint32_t> ids(n);
+    std::vector<float> lf(n), e(n);
+    std::vector<float4> pl(n), om(n);
+    HIP_TRY(hipMemcpy(ids.data(), s->B.ids[s->ic], sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(lf.data(), s->dbg_lam, sizeof(float) * n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(e.data(), s->dbg_err, sizeof(float) * n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(om.data(), s->B.omega, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    if (s->lam_last >= 0)
+        HIP_TRY(hipMemcpy(pl.data(), s->B.pos[s->lam_last], sizeof(float4) * n, hipMemcpyDeviceToHost));
+    for (int sl = 0; sl < n; ++sl) {
+        uint32_t id = ids[sl];
+        if (lam_first) lam_first[id] = lf[sl];
+        if (lam_last) lam_last[id] = s->lam_last >= 0 ? pl[sl].w : lf[sl];
+        if (err) err[id] = e[sl];
+        if (curl_xyz) {
+            curl_xyz[3 * (size_t)id] = om[sl].x;
+            curl_xyz[3 * (size_t)id + 1] = om[sl].y;
+            curl_xyz[3 * (size_t)id + 2] = om[sl].z;
+        }
+    }
+    return PBF_OK;
+}
+
+int pbf_density_stats(pbf_solver* s, float out[3]) {
+    if (!s || !out) return set_error(PBF_ERR_ARG, "NULL argument");
+    if (!s->stats_valid) return set_error(PBF_ERR_STATE, "no step with density statistics taken");
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    int nb = err_partial_blocks(std::max(s->n, 1));
+    std::vector<float> p((size_t)nb * 3);
+    HIP_TRY(hipMemcpy(p.data(), s->partial, sizeof(float) * p.size(), hipMemcpyDeviceToHost));
+    double sa = 0.0, sp = 0.0;
+    float mx = -INFINITY;
+    for (int b = 0; b < nb; ++b) {
+        sa += p[3 * b];
+        sp += p[3 * b + 1];
+        mx = std::max(mx, p[3 * b + 2]);
+    }
+    double n = std::max(s->n, 1);
+    out[0] = (float)(sa / n);
+    out[1] = (float)(sp / n);
+    out[2] = mx;
+    return PBF_OK;
+}
+
+int pbf_time_steps(pbf_solver* s, int32_t nsteps, float* ms_total) {
+    if (!s || !ms_total) return set_error(PBF_ERR_ARG, "NULL argument");
+    HIP_TRY(hipSetDevice(s->device));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, s->stream));
+    int rc = PBF_OK;
+    for (int i = 0; i < nsteps && rc == PBF_OK; ++i) rc = run_step(s);
+    HIP_TRY(hipEventRecord(e1, s->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_total = ms;
+    return rc;
+}
+
+int pbf_stage_times(pbf_solver* s, float out[7]) {
+    if (!s || !out) return set_error(PBF_ERR_ARG, "NULL argument");
+    HIP_TRY(hipSetDevice(s->device));
+    hipEvent_t ev[8];
+    for (int i = 0; i < 8; ++i) HIP_TRY(hipEventCreate(&ev[i]));
+    int rc = enqueue_step(s, ev);
+    for (int i = 0; i < 7; ++i) out[i] = 0.0f;
+    if (rc == PBF_OK) {
+        HIP_TRY(hipEventSynchronize(ev[7]));
+        for (int i = 0; i < 7; ++i) HIP_TRY(hipEventElapsedTime(&out[i], ev[i], ev[i + 1]));
+    }
+    for (int i = 0; i < 8; ++i) (void)hipEventDestroy(ev[i]);
+    return rc;
+}
+
+// One step, eager, with an event after every stage launch group; writes up to max entries of
+// (ms) into ms[] and the stage names (static strings) into names[]; returns the count.
+int pbf_profile_step(pbf_solver* s, float* ms, const char** names, int32_t max) {
+    if (!s || !ms || !names || max < 1) return set_error(PBF_ERR_ARG, "bad argument");
+    HIP_TRY(hipSetDevice(s->device));
+    std::vector<hipEvent_t> evs;
+    std::vector<const char*> nm;
+    s->prof_events = &evs;
+    s->prof_names = &nm;
+    int rc = enqueue_step(s, nullptr);
+    s->prof_events = nullptr;
+    s->prof_names = nullptr;
+    int cnt = 0;
+    if (rc == PBF_OK && !evs.empty()) {
+        HIP_TRY(hipEventSynchronize(evs.back()));
+        for (size_t i = 1; i < evs.size() && cnt < max; ++i, ++cnt) {
+            HIP_TRY(hipEventElapsedTime(&ms[cnt], evs[i - 1], evs[i]));
+            names[cnt] = nm[i];
+        }
+    }
+    for (hipEvent_t e : evs) (void)hipEventDestroy(e);
+    return rc == PBF_OK ? cnt : rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,117 @@
+// Device-side math of one PBF step: smoothing kernels, boundary clamp with jitter and grid
+// hashing.  The formulas are those of the reference's kernels/cudaPBFSolver.cuh:99-181; the
+// arithmetic is fixed so that a CPU build of the same operations gives the same bits:
+//   * fp32 +,-,*,/ and sqrt only (correctly rounded; the library is built with
+//     -ffp-contract=off and -fhip-fp32-correctly-rounded-divide-sqrt), left to right;
+//   * integer powers by repeated multiplication;
+//   * the jitter's sin is pbf_sin below, not the device libm's.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pbf {
+
+// Everything a step kernel needs, passed by value as one kernel argument.
+struct StepConsts {
+    float h, h2, c6, cs, wdq, k, scorr_n, restRho, eps, dt, visc_c, vort_eps, jitter, rmin;
+    int n_int;  // exponent of s_corr when it is a small non-negative integer, else -1
+    float g[3];
+    float gmin[3];
+    float invH;
+    int dims[3];
+    int ncells;  // dims[0]*dims[1]*dims[2]; bucket ncells holds out-of-grid particles
+    float bmin[3], bmax[3];
+    int cap;     // neighbour cap (the reference's MAX_NEIGHBOR = 63)
+    int n;       // particles
+    int nstride; // row stride of the neighbour table (>= n, multiple of 64)
+};
+
+// sin for the boundary jitter: Cody & Waite (1980) reduction by pi/2 in three parts
+// (P1: 8 significant bits, P2: 12) and the Taylor series of sin / cos on [-pi/4, pi/4],
+// truncated after x^9 / x^10.
+__device__ __forceinline__ float pbf_sin(float x) {
+    if (!(fabsf(x) < 1.0e6f)) return x - x;
+    const float two_over_pi = 0x1.45f306p-1f;
+    const float P1 = 0x1.92p+0f;
+    const float P2 = 0x1.fb6p-12f;
+    const float P3 = -0x1.777a5cp-25f;
+    float t = x * two_over_pi;
+    int k = (int)(t + (t >= 0.0f ? 0.5f : -0.5f));
+    float kf = (float)k;
+    float r = ((x - kf * P1) - kf * P2) - kf * P3;
+    float r2 = r * r;
+    float s = r + r * r2 * (-0x1.555556p-3f + r2 * (0x1.111112p-7f + r2 * (-0x1.a01a02p-13f + r2 * 0x1.71de3ap-19f)));
+    float c = 1.0f + r2 * (-0.5f + r2 * (0x1.555556p-5f + r2 * (-0x1.6c16c2p-10f + r2 * (0x1.a01a02p-16f + r2 * -0x1.27e4fcp-22f))));
+    float v = (k & 1) ? c : s;
+    return (k & 2) ? -v : v;
+}
+
+// frac / randomOffset, cuh:112-119.
+__device__ __forceinline__ float random_offset(float u, float v) {
+    float a = pbf_sin(2.0f * (u * 12.9898f + v * 78.233f)) * 43758.5453f;
+    return fabsf(a - (float)(int)a);
+}
+
+// BoundParticle, cuh:121-140, including the y-max case taking its jitter from (y, z).
+__device__ __forceinline__ void bound_particle(float& x, float& y, float& z, const StepConsts& K) {
+    if (x < K.bmin[0]) x = K.bmin[0] + K.jitter * random_offset(y, z);
+    if (x > K.bmax[0]) x = K.bmax[0] - K.jitter * random_offset(y, z);
+    if (y < K.bmin[1]) y = K.bmin[1] + K.jitter * random_offset(x, z);
+    if (y > K.bmax[1]) y = K.bmax[1] - K.jitter * random_offset(y, z);
+    if (z < K.bmin[2]) z = K.bmin[2] + K.jitter * random_offset(x, y);
+    if (z > K.bmax[2]) z = K.bmax[2] - K.jitter * random_offset(x, y);
+}
+
+// Wpoly6, cuh:99-103, on r.r.
+__device__ __forceinline__ float poly6(float r2, const StepConsts& K) {
+    if (r2 > K.h2) return 0.0f;
+    float d = K.h2 - r2;
+    return K.c6 * (d * d * d);
+}
+
+// WspikyGrad, cuh:105-110, returned as the scalar t with grad = r * t.
+__device__ __forceinline__ float spiky_scale(float r2, const StepConsts& K) {
+    float len = sqrtf(r2);
+    if (len > K.h || len < K.rmin) return 0.0f;
+    float hl = K.h - len;
+    return (K.cs * (hl * hl)) / len;
+}
+
+__device__ __forceinline__ float powi(float b, int e) {
+    float r = 1.0f;
+    while (e) {
+        if (e & 1) r = r * b;
+        b = b * b;
+        e >>= 1;
+    }
+    return r;
+}
+
+// Scorr, cuh:142-144, given W(r).
+__device__ __forceinline__ float scorr(float w, const StepConsts& K) {
+    float ratio = w / K.wdq;
+    float p = K.n_int >= 0 ? powi(ratio, K.n_int) : powf(ratio, K.scorr_n);
+    return -K.k * p;
+}
+
+// computeDimID, cuh:156-160: truncation toward zero of (p - pMin) * invH, clamped to
+// [-3, dim+2] first (no in-range cell or 27-cell search changes), NaN to a far cell.
+__device__ __forceinline__ int dim_of(float p, float pmin, float invH, int dim) {
+    float f = (p - pmin) * invH;
+    if (!(f == f)) return -0x20000000;
+    f = f < -3.0f ? -3.0f : f;
+    f = f > (float)(dim + 2) ? (float)(dim + 2) : f;
+    return (int)f;
+}
+
+// computeGid, cuh:172-181; out-of-grid particles go to bucket ncells.
+__device__ __forceinline__ int cell_of(float x, float y, float z, const StepConsts& K) {
+    int d0 = dim_of(x, K.gmin[0], K.invH, K.dims[0]);
+    int d1 = dim_of(y, K.gmin[1], K.invH, K.dims[1]);
+    int d2 = dim_of(z, K.gmin[2], K.invH, K.dims[2]);
+    bool out = d0 < 0 || d0 >= K.dims[0] || d1 < 0 || d1 >= K.dims[1] || d2 < 0 || d2 >= K.dims[2];
+    return out ? K.ncells : d0 + d1 * K.dims[0] + d2 * K.dims[0] * K.dims[1];
+}
+
+}  // namespace pbf

@@ -1,0 +1,56 @@
+// Launch interface of the step kernels (pbf_kernels.hip) for the host runtime.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pbf_device.h"
+
+namespace pbf {
+
+// Device buffers of one solver; index [2] arrays alternate between steps / iterations.
+struct StepBuffers {
+    float4* pos[2];
+    float4* vel[2];
+    uint32_t* ids[2];
+    float4* xold;
+    float4* xp;
+    float4* vp;
+    float4* omega;
+    uint32_t* cell;
+    uint32_t* local;
+    uint32_t* count;     // ncells + 1 (+ padding to 4)
+    uint32_t* cstart;    // ncells + 2
+    uint32_t* tile_sum;  // scan tiles
+    uint32_t* bucket;
+    uint32_t* nbr;       // cap * nstride
+    uint8_t* ncnt;
+};
+
+int scan_tiles(int m);
+int err_partial_blocks(int n);
+
+// predict + clamp + hash + count (reads the state, writes xp / vp / cell / local / count)
+void launch_predict_hash(hipStream_t st, const StepConsts& K, const float4* pos, const float4* vel,
+                         const StepBuffers& B);
+// exclusive scan of count into cstart; clears count for the next step
+void launch_scan(hipStream_t st, const StepConsts& K, const StepBuffers& B);
+// scatter into buckets, rank by particle id, move xp / vp / start-of-step position / id to slot order
+void launch_sort_reorder(hipStream_t st, const StepConsts& K, const StepBuffers& B, const float4* pos_in,
+                         const uint32_t* ids_in, float4* pos_out, float4* vel_out, uint32_t* ids_out,
+                         int32_t* dbg_gid);
+void launch_neighbors_lambda(hipStream_t st, const StepConsts& K, float4* pos, const StepBuffers& B, float* dbg_lam);
+void launch_lambda(hipStream_t st, const StepConsts& K, float4* pos, const StepBuffers& B);
+void launch_delta_apply(hipStream_t st, const StepConsts& K, const float4* pos, float4* pos_next, float4* vel,
+                        const StepBuffers& B, int last);
+void launch_curl(hipStream_t st, const StepConsts& K, const float4* pos, const float4* vel, const StepBuffers& B,
+                 float* err, float* partial);
+void launch_vort_apply(hipStream_t st, const StepConsts& K, const float4* pos, const float4* vel, float4* vel_out,
+                       const StepBuffers& B);
+void launch_export(hipStream_t st, int n, const float4* pos, const float4* vel, const uint32_t* ids, float* pos_xyz,
+                   float* vel_xyz);
+void launch_import(hipStream_t st, int n, const float* pos_xyz, const float* vel_xyz, float4* pos, float4* vel,
+                   uint32_t* ids);
+void launch_fill_u32(hipStream_t st, uint32_t* p, size_t m, uint32_t v);
+
+}  // namespace pbf

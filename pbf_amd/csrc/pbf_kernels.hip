@@ -1,0 +1,543 @@
+// gfx950 kernels of one PBF solver step.
+//
+// Data layout in HBM (all per-particle arrays in "slot" order, the cell-sorted order of the
+// current step; ids[slot] is the caller's particle index):
+//   pos4   float4 (x, y, z, lambda)   double-buffered across Jacobi iterations
+//   vel4   float4 (vx, vy, vz, -)      double-buffered across the vorticity pass
+//   xold4  float4 (x_old)             position at the start of the step
+//   omega4 float4 (wx, wy, wz, |w|)   curl
+//   nbr    uint32 [cap][nstride]       neighbour slots, row j holds every particle's j-th
+//                                     neighbour (coalesced across a wavefront)
+//   ncnt   uint8  [n]                  neighbour count
+//   count / cstart  uint32 [ncells+1] / [ncells+2]  cell histogram and its exclusive scan;
+//                                     bucket ncells collects out-of-grid particles
+//
+// Reference kernels each stage replaces are cited at the kernel.
+
+#include "pbf_device.h"
+#include "pbf_kernels.h"
+
+namespace pbf {
+
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------------------
+// wavefront (64-lane) inclusive scan with DPP: Hillis-Steele inside each row of 16 lanes
+// (row_shr 1, 2, 4, 8 with zero fill), then row_bcast:15 and row_bcast:31 carry the row
+// totals across rows.
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);
+    return x;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
+    x = wave_inclusive_scan(x);
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+}
+
+// ---------------------------------------------------------------------------------------
+// updateParticles (cu:4-13) + calculateParticlesGid (cu:31-40): predict, clamp, hash, count.
+// The returned atomic slot gives an (unordered) place inside the cell bucket.
+__global__ __launch_bounds__(kBlock) void k_predict_hash(StepConsts K, const float4* __restrict__ pos,
+                                                         const float4* __restrict__ vel, float4* __restrict__ xp,
+                                                         float4* __restrict__ vp, uint32_t* __restrict__ cell,
+                                                         uint32_t* __restrict__ local, uint32_t* __restrict__ count) {
+    int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= K.n) return;
+    float4 x = pos[s];
+    float4 v = vel[s];
+    v.x = v.x + K.dt * K.g[0];
+    v.y = v.y + K.dt * K.g[1];
+    v.z = v.z + K.dt * K.g[2];
+    x.x = x.x + K.dt * v.x;
+    x.y = x.y + K.dt * v.y;
+    x.z = x.z + K.dt * v.z;
+    bound_particle(x.x, x.y, x.z, K);
+    int c = cell_of(x.x, x.y, x.z, K);
+    uint32_t l = atomicAdd(&count[c], 1u);
+    xp[s] = make_float4(x.x, x.y, x.z, 0.0f);
+    vp[s] = make_float4(v.x, v.y, v.z, 0.0f);
+    cell[s] = (uint32_t)c;
+    local[s] = l;
+}
+
+// ---------------------------------------------------------------------------------------
+// Exclusive scan of count[0..m) into cstart[0..m], cstart[m] = total (calculatePrefixSum +
+// addBlockSum + setGrid, cu:42-111).  Reduce-then-scan over tiles of 1024 counters: tile sums,
+// one workgroup scanning the tile sums, then every tile rescanned with its offset.  The last
+// pass also clears the counters for the next step.
+constexpr int kScanTile = 1024;
+
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ count, int m,
+                                                        uint32_t* __restrict__ tile_sum) {
+    __shared__ uint32_t wsum[kBlock / 64];
+    int base = blockIdx.x * kScanTile + threadIdx.x * 4;
+    uint32_t t = 0;
+    if (base + 3 < m) {
+        uint4 c = *reinterpret_cast<const uint4*>(count + base);
+        t = c.x + c.y + c.z + c.w;
+    } else {
+        for (int j = 0; j < 4; ++j)
+            if (base + j < m) t += count[base + j];
+    }
+    t = wave_sum(t);
+    int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) wsum[wave] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) tile_sum[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_tiles(uint32_t* __restrict__ tile_sum, int ntiles,
+                                                     uint32_t* __restrict__ cstart, int m) {
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry_s;
+    if (threadIdx.x == 0) carry_s = 0;
+    __syncthreads();
+    for (int base = 0; base < ntiles; base += 1024) {
+        int i = base + threadIdx.x;
+        uint32_t v = i < ntiles ? tile_sum[i] : 0u;
+        uint32_t inc = wave_inclusive_scan(v);
+        int wave = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 63) wsum[wave] = inc;
+        __syncthreads();
+        uint32_t off = carry_s;
+        for (int w = 0; w < wave; ++w) off += wsum[w];
+        if (i < ntiles) tile_sum[i] = off + inc - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry_s = off + inc;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) cstart[m] = carry_s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_down(uint32_t* __restrict__ count, int m,
+                                                      const uint32_t* __restrict__ tile_off,
+                                                      uint32_t* __restrict__ cstart) {
+    __shared__ uint32_t wsum[kBlock / 64];
+    int base = blockIdx.x * kScanTile + threadIdx.x * 4;
+    uint32_t c[4];
+    bool full = base + 3 < m;
+    if (full) {
+        uint4 q = *reinterpret_cast<const uint4*>(count + base);
+        c[0] = q.x;
+        c[1] = q.y;
+        c[2] = q.z;
+        c[3] = q.w;
+    } else {
+        for (int j = 0; j < 4; ++j) c[j] = base + j < m ? count[base + j] : 0u;
+    }
+    uint32_t t = c[0] + c[1] + c[2] + c[3];
+    uint32_t inc = wave_inclusive_scan(t);
+    int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 63) wsum[wave] = inc;
+    __syncthreads();
+    uint32_t off = tile_off[blockIdx.x];
+    for (int w = 0; w < wave; ++w) off += wsum[w];
+    off += inc - t;
+    uint32_t o[4];
+    o[0] = off;
+    o[1] = o[0] + c[0];
+    o[2] = o[1] + c[1];
+    o[3] = o[2] + c[2];
+    if (full) {
+        *reinterpret_cast<uint4*>(cstart + base) = make_uint4(o[0], o[1], o[2], o[3]);
+        *reinterpret_cast<uint4*>(count + base) = make_uint4(0u, 0u, 0u, 0u);
+    } else {
+        for (int j = 0; j < 4; ++j)
+            if (base + j < m) {
+                cstart[base + j] = o[j];
+                count[base + j] = 0u;
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// countingSort (cu:112-122) scatter into cell buckets, unordered inside a bucket.
+__global__ __launch_bounds__(kBlock) void k_scatter(int n, const uint32_t* __restrict__ cell,
+                                                    const uint32_t* __restrict__ local,
+                                                    const uint32_t* __restrict__ cstart, uint32_t* __restrict__ bucket) {
+    int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    bucket[cstart[cell[s]] + local[s]] = (uint32_t)s;
+}
+
+// Orders each bucket by ascending particle id (rank = members with a smaller id) and moves the
+// particle's predicted position, velocity, start-of-step position and id to its final slot.
+// Out-of-grid particles keep their bucket order: nothing reads their relative order.
+__global__ __launch_bounds__(kBlock) void k_rank_reorder(StepConsts K, const uint32_t* __restrict__ bucket,
+                                                         const uint32_t* __restrict__ cell,
+                                                         const uint32_t* __restrict__ cstart,
+                                                         const uint32_t* __restrict__ ids_in,
+                                                         const float4* __restrict__ xp, const float4* __restrict__ vp,
+                                                         const float4* __restrict__ pos_in, float4* __restrict__ pos_out,
+                                                         float4* __restrict__ vel_out, float4* __restrict__ xold_out,
+                                                         uint32_t* __restrict__ ids_out, int32_t* __restrict__ dbg_gid) {
+    int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= K.n) return;
+    uint32_t e = bucket[t];
+    uint32_t c = cell[e];
+    uint32_t id = ids_in[e];
+    int d = t;
+    if ((int)c < K.ncells) {
+        uint32_t st = cstart[c], en = cstart[c + 1];
+        uint32_t rank = 0;
+        for (uint32_t q = st; q < en; ++q) rank += ids_in[bucket[q]] < id ? 1u : 0u;
+        d = (int)(st + rank);
+    }
+    pos_out[d] = xp[e];
+    vel_out[d] = vp[e];
+    xold_out[d] = pos_in[e];
+    ids_out[d] = id;
+    if (dbg_gid) dbg_gid[id] = (int)c < K.ncells ? (int32_t)c : -1;
+}
+
+// ---------------------------------------------------------------------------------------
+// Per-pair lambda terms (calculateLambda, cu:156-180).
+struct LambdaAcc {
+    float rho, gms, gx, gy, gz;
+    __device__ __forceinline__ void init() { rho = gms = gx = gy = gz = 0.0f; }
+    __device__ __forceinline__ void add(float rx, float ry, float rz, float r2, const StepConsts& K) {
+        rho = rho + poly6(r2, K);
+        float t = spiky_scale(r2, K);
+        float ax = rx * t, ay = ry * t, az = rz * t;
+        gx = gx + ax;
+        gy = gy + ay;
+        gz = gz + az;
+        gms = gms + (ax * ax + ay * ay + az * az);
+    }
+    __device__ __forceinline__ float lambda(const StepConsts& K) const {
+        float Ci = rho / K.restRho - 1.0f;
+        float g = gms + (gx * gx + gy * gy + gz * gz);
+        g = g / (K.restRho * K.restRho);
+        return -Ci / (g + K.eps);
+    }
+};
+
+// buildNeighborhood (cu:124-154) fused with the first calculateLambda: the 27 cells in z, y, x
+// order (the three x-neighbours of a row are one contiguous slot run), self excluded,
+// r.r <= h*h, at most cap hits.  Lambda of iteration 1 goes to pos.w.
+__global__ __launch_bounds__(kBlock) void k_neighbors_lambda(StepConsts K, float4* __restrict__ pos,
+                                                             const uint32_t* __restrict__ cstart,
+                                                             uint32_t* __restrict__ nbr, uint8_t* __restrict__ ncnt,
+                                                             float* __restrict__ dbg_lam) {
+    int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= K.n) return;
+    const float4 xi = pos[s];
+    int d0 = dim_of(xi.x, K.gmin[0], K.invH, K.dims[0]);
+    int d1 = dim_of(xi.y, K.gmin[1], K.invH, K.dims[1]);
+    int d2 = dim_of(xi.z, K.gmin[2], K.invH, K.dims[2]);
+    int xlo = max(d0 - 1, 0), xhi = min(d0 + 1, K.dims[0] - 1);
+    LambdaAcc A;
+    A.init();
+    int cnt = 0;
+    uint32_t* out = nbr + s;
+    for (int dz = -1; dz <= 1; ++dz) {
+        int cz = d2 + dz;
+        if (cz < 0 || cz >= K.dims[2]) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            int cy = d1 + dy;
+            if (cy < 0 || cy >= K.dims[1] || xlo > xhi) continue;
+            int row = (cz * K.dims[1] + cy) * K.dims[0];
+            uint32_t b = cstart[row + xlo], e = cstart[row + xhi + 1];
+            for (uint32_t q = b; q < e && cnt < K.cap; ++q) {
+                if ((int)q == s) continue;
+                float4 xj = pos[q];
+                float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+                float r2 = rx * rx + ry * ry + rz * rz;
+                if (r2 <= K.h2) {
+                    out[(size_t)cnt * K.nstride] = q;
+                    ++cnt;
+                    A.add(rx, ry, rz, r2, K);
+                }
+            }
+        }
+    }
+    ncnt[s] = (uint8_t)cnt;
+    float lam = A.lambda(K);
+    reinterpret_cast<float*>(pos + s)[3] = lam;
+    if (dbg_lam) dbg_lam[s] = lam;
+}
+
+// calculateLambda (cu:156-180) over the frozen list, iterations 2..I.
+__global__ __launch_bounds__(kBlock) void k_lambda(StepConsts K, float4* __restrict__ pos,
+                                                   const uint32_t* __restrict__ nbr,
+                                                   const uint8_t* __restrict__ ncnt) {
+    int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= K.n) return;
+    const float4 xi = pos[s];
+    int cnt = ncnt[s];
+    LambdaAcc A;
+    A.init();
+    const uint32_t* L = nbr + s;
+    for (int j = 0; j < cnt; ++j) {
+        float4 xj = pos[L[(size_t)j * K.nstride]];
+        float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        float r2 = rx * rx + ry * ry + rz * rz;
+        A.add(rx, ry, rz, r2, K);
+    }
+    reinterpret_cast<float*>(pos + s)[3] = A.lambda(K);
+}
+
+// calculateOffset (cu:182-199) + updatePosition (cu:15-22): delta-p with s_corr, then
+// x + delta-p clamped into the next position buffer; the last iteration also writes
+// v = (x - x_old) / dt.
+__global__ __launch_bounds__(kBlock) void k_delta_apply(StepConsts K, const float4* __restrict__ pos,
+                                                        float4* __restrict__ pos_next,
+                                                        const uint32_t* __restrict__ nbr,
+                                                        const uint8_t* __restrict__ ncnt,
+                                                        const float4* __restrict__ xold, float4* __restrict__ vel,
+                                                        int last) {
+    int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= K.n) return;
+    const float4 xi = pos[s];
+    int cnt = ncnt[s];
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    const uint32_t* L = nbr + s;
+    for (int j = 0; j < cnt; ++j) {
+        float4 xj = pos[L[(size_t)j * K.nstride]];
+        float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        float r2 = rx * rx + ry * ry + rz * rz;
+        float w = poly6(r2, K);
+        float t = spiky_scale(r2, K);
+        float a = (xi.w + xj.w) + scorr(w, K);
+        ax = ax + a * (rx * t);
+        ay = ay + a * (ry * t);
+        az = az + a * (rz * t);
+    }
+    float x = xi.x + ax / K.restRho;
+    float y = xi.y + ay / K.restRho;
+    float z = xi.z + az / K.restRho;
+    bound_particle(x, y, z, K);
+    pos_next[s] = make_float4(x, y, z, 0.0f);
+    if (last) {
+        float4 o = xold[s];
+        vel[s] = make_float4((x - o.x) / K.dt, (y - o.y) / K.dt, (z - o.z) / K.dt, 0.0f);
+    }
+}
+
+// Block reduction of the density-error statistics (sum |e|, sum max(e,0), max e) in a fixed
+// tree order.
+__device__ void block_err_stats(float e, bool valid, float* __restrict__ partial) {
+    __shared__ float sa[kBlock], sp[kBlock], sm[kBlock];
+    int t = threadIdx.x;
+    sa[t] = valid ? fabsf(e) : 0.0f;
+    sp[t] = valid ? fmaxf(e, 0.0f) : 0.0f;
+    sm[t] = valid ? e : -INFINITY;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            sa[t] = sa[t] + sa[t + w];
+            sp[t] = sp[t] + sp[t + w];
+            sm[t] = fmaxf(sm[t], sm[t + w]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        partial[blockIdx.x * 3 + 0] = sa[0];
+        partial[blockIdx.x * 3 + 1] = sp[0];
+        partial[blockIdx.x * 3 + 2] = sm[0];
+    }
+}
+
+// calculateCurl (cu:201-215), plus the density error rho/rho0 - 1 at the final positions when
+// err or partial is given.
+__global__ __launch_bounds__(kBlock) void k_curl(StepConsts K, const float4* __restrict__ pos,
+                                                 const float4* __restrict__ vel, const uint32_t* __restrict__ nbr,
+                                                 const uint8_t* __restrict__ ncnt, float4* __restrict__ omega,
+                                                 float* __restrict__ err, float* __restrict__ partial) {
+    int s = blockIdx.x * kBlock + threadIdx.x;
+    bool valid = s < K.n;
+    float e = 0.0f;
+    if (valid) {
+        const float4 xi = pos[s];
+        const float4 vi = vel[s];
+        int cnt = ncnt[s];
+        float wx = 0.0f, wy = 0.0f, wz = 0.0f, rho = 0.0f;
+        const uint32_t* L = nbr + s;
+        for (int j = 0; j < cnt; ++j) {
+            uint32_t q = L[(size_t)j * K.nstride];
+            float4 xj = pos[q];
+            float4 vj = vel[q];
+            float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
+            float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+            float r2 = rx * rx + ry * ry + rz * rz;
+            float t = spiky_scale(r2, K);
+            float gx = rx * t, gy = ry * t, gz = rz * t;
+            wx = wx + (uy * gz - uz * gy);
+            wy = wy + (uz * gx - ux * gz);
+            wz = wz + (ux * gy - uy * gx);
+            rho = rho + poly6(r2, K);
+        }
+        omega[s] = make_float4(wx, wy, wz, sqrtf(wx * wx + wy * wy + wz * wz));
+        e = rho / K.restRho - 1.0f;
+        if (err) err[s] = e;
+    }
+    if (partial) block_err_stats(e, valid, partial);
+}
+
+// calculateVorticityAndViscosity (cu:217-239) + applyVorticityAndViscosity (cu:24-29).
+__global__ __launch_bounds__(kBlock) void k_vort_apply(StepConsts K, const float4* __restrict__ pos,
+                                                       const float4* __restrict__ vel,
+                                                       const float4* __restrict__ omega,
+                                                       const uint32_t* __restrict__ nbr,
+                                                       const uint8_t* __restrict__ ncnt, float4* __restrict__ vel_out) {
+    int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= K.n) return;
+    const float4 xi = pos[s];
+    const float4 vi = vel[s];
+    const float4 wi = omega[s];
+    int cnt = ncnt[s];
+    float px = 0.0f, py = 0.0f, pz = 0.0f, ex = 0.0f, ey = 0.0f, ez = 0.0f;
+    const uint32_t* L = nbr + s;
+    for (int j = 0; j < cnt; ++j) {
+        uint32_t q = L[(size_t)j * K.nstride];
+        float4 xj = pos[q];
+        float4 vj = vel[q];
+        float wl = omega[q].w;
+        float ux = vi.x - vj.x, uy = vi.y - vj.y, uz = vi.z - vj.z;
+        float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        float r2 = rx * rx + ry * ry + rz * rz;
+        float w = poly6(r2, K);
+        px = px - ux * w;
+        py = py - uy * w;
+        pz = pz - uz * w;
+        float t = spiky_scale(r2, K);
+        ex = ex + wl * (rx * t);
+        ey = ey + wl * (ry * t);
+        ez = ez + wl * (rz * t);
+    }
+    float el = sqrtf(ex * ex + ey * ey + ez * ez);
+    if ((double)fabsf(el) > 1e-3) {
+        ex = ex / el;
+        ey = ey / el;
+        ez = ez / el;
+    }
+    float cx = ey * wi.z - ez * wi.y;
+    float cy = ez * wi.x - ex * wi.z;
+    float cz = ex * wi.y - ey * wi.x;
+    float fx = K.vort_eps * cx, fy = K.vort_eps * cy, fz = K.vort_eps * cz;
+    float vx = K.visc_c * px, vy = K.visc_c * py, vz = K.visc_c * pz;
+    vel_out[s] = make_float4(vi.x + (K.dt * fx + vx), vi.y + (K.dt * fy + vy), vi.z + (K.dt * fz + vz), 0.0f);
+}
+
+// ---------------------------------------------------------------------------------------
+// State in / out in the caller's particle order (AoS xyz).
+__global__ __launch_bounds__(kBlock) void k_export(int n, const float4* __restrict__ pos,
+                                                   const float4* __restrict__ vel, const uint32_t* __restrict__ ids,
+                                                   float* __restrict__ pos_xyz, float* __restrict__ vel_xyz) {
+    int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    uint32_t i = ids[s];
+    if (pos_xyz) {
+        float4 p = pos[s];
+        pos_xyz[3 * (size_t)i + 0] = p.x;
+        pos_xyz[3 * (size_t)i + 1] = p.y;
+        pos_xyz[3 * (size_t)i + 2] = p.z;
+    }
+    if (vel_xyz) {
+        float4 v = vel[s];
+        vel_xyz[3 * (size_t)i + 0] = v.x;
+        vel_xyz[3 * (size_t)i + 1] = v.y;
+        vel_xyz[3 * (size_t)i + 2] = v.z;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_import(int n, const float* __restrict__ pos_xyz,
+                                                   const float* __restrict__ vel_xyz, float4* __restrict__ pos,
+                                                   float4* __restrict__ vel, uint32_t* __restrict__ ids) {
+    int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    pos[s] = make_float4(pos_xyz[3 * (size_t)s], pos_xyz[3 * (size_t)s + 1], pos_xyz[3 * (size_t)s + 2], 0.0f);
+    vel[s] = make_float4(vel_xyz[3 * (size_t)s], vel_xyz[3 * (size_t)s + 1], vel_xyz[3 * (size_t)s + 2], 0.0f);
+    ids[s] = (uint32_t)s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ p, size_t m, uint32_t v) {
+    size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < m) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------------------
+// launch wrappers
+static inline unsigned nblk(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+int scan_tiles(int m) { return (m + kScanTile - 1) / kScanTile; }
+
+void launch_predict_hash(hipStream_t st, const StepConsts& K, const float4* pos, const float4* vel,
+                         const StepBuffers& B) {
+    if (K.n > 0)
+        hipLaunchKernelGGL(k_predict_hash, dim3(nblk(K.n)), dim3(kBlock), 0, st, K, pos, vel, B.xp, B.vp, B.cell,
+                           B.local, B.count);
+}
+
+void launch_scan(hipStream_t st, const StepConsts& K, const StepBuffers& B) {
+    int m = K.ncells + 1;
+    int nt = scan_tiles(m);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nt), dim3(kBlock), 0, st, B.count, m, B.tile_sum);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, st, B.tile_sum, nt, B.cstart, m);
+    hipLaunchKernelGGL(k_scan_down, dim3(nt), dim3(kBlock), 0, st, B.count, m, B.tile_sum, B.cstart);
+}
+
+void launch_sort_reorder(hipStream_t st, const StepConsts& K, const StepBuffers& B, const float4* pos_in,
+                         const uint32_t* ids_in, float4* pos_out, float4* vel_out, uint32_t* ids_out,
+                         int32_t* dbg_gid) {
+    int n = K.n;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(n)), dim3(kBlock), 0, st, n, B.cell, B.local, B.cstart, B.bucket);
+    hipLaunchKernelGGL(k_rank_reorder, dim3(nblk(n)), dim3(kBlock), 0, st, K, B.bucket, B.cell, B.cstart, ids_in,
+                       B.xp, B.vp, pos_in, pos_out, vel_out, B.xold, ids_out, dbg_gid);
+}
+
+void launch_neighbors_lambda(hipStream_t st, const StepConsts& K, float4* pos, const StepBuffers& B, float* dbg_lam) {
+    if (K.n > 0)
+        hipLaunchKernelGGL(k_neighbors_lambda, dim3(nblk(K.n)), dim3(kBlock), 0, st, K, pos, B.cstart, B.nbr, B.ncnt,
+                           dbg_lam);
+}
+
+void launch_lambda(hipStream_t st, const StepConsts& K, float4* pos, const StepBuffers& B) {
+    if (K.n > 0) hipLaunchKernelGGL(k_lambda, dim3(nblk(K.n)), dim3(kBlock), 0, st, K, pos, B.nbr, B.ncnt);
+}
+
+void launch_delta_apply(hipStream_t st, const StepConsts& K, const float4* pos, float4* pos_next, float4* vel,
+                        const StepBuffers& B, int last) {
+    if (K.n > 0)
+        hipLaunchKernelGGL(k_delta_apply, dim3(nblk(K.n)), dim3(kBlock), 0, st, K, pos, pos_next, B.nbr, B.ncnt,
+                           B.xold, vel, last);
+}
+
+void launch_curl(hipStream_t st, const StepConsts& K, const float4* pos, const float4* vel, const StepBuffers& B,
+                 float* err, float* partial) {
+    if (K.n > 0)
+        hipLaunchKernelGGL(k_curl, dim3(nblk(K.n)), dim3(kBlock), 0, st, K, pos, vel, B.nbr, B.ncnt, B.omega, err,
+                           partial);
+}
+
+void launch_vort_apply(hipStream_t st, const StepConsts& K, const float4* pos, const float4* vel, float4* vel_out,
+                       const StepBuffers& B) {
+    if (K.n > 0)
+        hipLaunchKernelGGL(k_vort_apply, dim3(nblk(K.n)), dim3(kBlock), 0, st, K, pos, vel, B.omega, B.nbr, B.ncnt,
+                           vel_out);
+}
+
+int err_partial_blocks(int n) { return (int)nblk(n); }
+
+void launch_export(hipStream_t st, int n, const float4* pos, const float4* vel, const uint32_t* ids, float* pos_xyz,
+                   float* vel_xyz) {
+    if (n > 0) hipLaunchKernelGGL(k_export, dim3(nblk(n)), dim3(kBlock), 0, st, n, pos, vel, ids, pos_xyz, vel_xyz);
+}
+
+void launch_import(hipStream_t st, int n, const float* pos_xyz, const float* vel_xyz, float4* pos, float4* vel,
+                   uint32_t* ids) {
+    if (n > 0) hipLaunchKernelGGL(k_import, dim3(nblk(n)), dim3(kBlock), 0, st, n, pos_xyz, vel_xyz, pos, vel, ids);
+}
+
+void launch_fill_u32(hipStream_t st, uint32_t* p, size_t m, uint32_t v) {
+    if (m > 0) hipLaunchKernelGGL(k_fill_u32, dim3(nblk(m)), dim3(kBlock), 0, st, p, m, v);
+}
+
+}  // namespace pbf

@@ -1,0 +1,202 @@
+"""GPU parity: the HIP solver (through the C ABI) against the oracle on identical inputs.
+
+The kernels and the oracle follow one arithmetic contract (IEEE fp32 +,-,*,/,sqrt, left to
+right, no contraction, a shared self-written sin for the clamp jitter), so everything is
+compared bit for bit: cell ids, counts, cell starts, sorted order, neighbour lists, lambda,
+curl, density error and the state after one and after many steps.
+"""
+import numpy as np
+import pytest
+
+from oracle_lib import Oracle
+from pbf_amd import FLAG_DEBUG, FLAG_DENSITY_STATS, FLAG_GRAPH, Solver, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def assert_bitexact(a, b, what):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    same = (bits(a) == bits(b)) | (np.isnan(a) & np.isnan(b))
+    if not same.all():
+        idx = np.argwhere(~same)[:5]
+        diff = np.nanmax(np.abs(a.astype(np.float64) - b.astype(np.float64)))
+        raise AssertionError(f"{what}: {int((~same).sum())} of {same.size} differ, max |d| = {diff}, first {idx.tolist()}")
+
+
+def pair(scene, flags=FLAG_DEBUG, **over):
+    print(f"[case] {scene.name} n={scene.n} over={over} flags={flags}", flush=True)
+    return Solver.from_scene(scene, flags=flags, **over), Oracle.from_scene(scene, **over)
+
+
+def compare_step(g, o, check_lists=True):
+    gg = g.grid()
+    og = o.grid()
+    nin = og[4]
+    assert gg[4] == nin, "particles inside the grid"
+    assert np.array_equal(gg[0], og[0]), "cell id per particle"
+    assert np.array_equal(gg[1], og[1]), "cell counts"
+    assert np.array_equal(gg[2], og[2]), "cell starts (exclusive scan)"
+    assert np.array_equal(gg[3][:nin], og[3][:nin]), "sorted order"
+    if check_lists:
+        gn, gc = g.neighbors()
+        on, oc = o.neighbors()
+        assert np.array_equal(gc, oc), "neighbour counts"
+        assert np.array_equal(gn, on), "neighbour lists"
+    gf = g.fields()
+    of = o.fields()
+    for name, a, b in zip(("lambda first", "lambda last", "density error", "curl"), gf, of):
+        assert_bitexact(a, b, name)
+    gp, gv = g.get_state()
+    op, ov = o.get_state()
+    assert_bitexact(gp, op, "positions")
+    assert_bitexact(gv, ov, "velocities")
+
+
+def test_one_step_32k():
+    g, o = pair(scenes.config_cpu_32k())
+    g.step()
+    o.step(1)
+    compare_step(g, o)
+
+
+def test_one_step_262k():
+    g, o = pair(scenes.config_262k())
+    g.step()
+    o.step(1)
+    compare_step(g, o)
+
+
+def test_one_step_1m():
+    g, o = pair(scenes.config_1m())
+    g.step()
+    o.step(1)
+    compare_step(g, o, check_lists=False)
+
+
+def test_many_steps_32k_bitexact():
+    g, o = pair(scenes.config_cpu_32k(), flags=FLAG_DENSITY_STATS)
+    for _ in range(4):
+        g.advance(25)
+        o.step(25)
+        gp, gv = g.get_state()
+        op, ov = o.get_state()
+        assert_bitexact(gp, op, "positions")
+        assert_bitexact(gv, ov, "velocities")
+
+
+def test_graph_replay_matches_eager():
+    sc = scenes.dam_break((24, 24, 24), (0, 0, 0, 1.6, 1.6, 1.6), seed=5)
+    a = Solver.from_scene(sc, flags=0)
+    b = Solver.from_scene(sc, flags=FLAG_GRAPH)
+    for _ in range(3):
+        a.advance(7)
+        b.advance(7)
+        pa, va = a.get_state()
+        pb, vb = b.get_state()
+        assert_bitexact(pa, pb, "graph positions")
+        assert_bitexact(va, vb, "graph velocities")
+
+
+def test_literal_parameters_one_step():
+    """The physical constants passed unchanged (a diverging configuration) still match."""
+    sc = scenes.config_cpu_32k()
+    lit = scenes.literal_params()
+    g, o = pair(sc, **lit)
+    for _ in range(3):
+        g.step()
+        o.step(1)
+    compare_step(g, o)
+
+
+@pytest.mark.parametrize("iters", [0, 1, 3])
+def test_iteration_counts(iters):
+    g, o = pair(scenes.dam_break((16, 16, 16), (0, 0, 0, 1.6, 1.6, 1.6), seed=2), iterations=iters)
+    g.step()
+    o.step(1)
+    compare_step(g, o)
+
+
+def test_reference_grid_64_cubed():
+    """The reference's literal 64^3 grid (solverBase.h:11-13) anchored at the box minimum."""
+    sc = scenes.dam_break((32, 32, 32), (0, 0, 0, 6.4, 6.4, 6.4), seed=3)
+    assert sc.grid_dims == (64, 64, 64)
+    g, o = pair(sc)
+    g.step()
+    o.step(1)
+    compare_step(g, o)
+
+
+def test_out_of_grid_and_nan():
+    """Particles outside the grid (the reference's gid -1, cu:37/118/141) and a NaN particle."""
+    sc = scenes.dam_break((12, 12, 12), (0, 0, 0, 1.6, 1.6, 1.6), seed=4)
+    pos = sc.pos.copy()
+    # grid smaller than the box: particles above y = 0.8 hash outside
+    sc.grid_dims = (16, 8, 16)
+    pos[5] = [np.nan, 0.5, 0.5]
+    sc.pos = pos
+    g, o = pair(sc)
+    g.step()
+    o.step(1)
+    gg, og = g.grid(), o.grid()
+    assert og[4] < sc.n
+    compare_step(g, o)
+
+
+def test_neighbour_cap_cluster():
+    """A dense cluster exceeds the 63-neighbour cap (cu:142)."""
+    rng = np.random.default_rng(0)
+    sc = scenes.dam_break((10, 10, 10), (0, 0, 0, 1.6, 1.6, 1.6), seed=6)
+    pos = sc.pos.copy()
+    pos[:200] = (np.array([0.8, 0.8, 0.8]) + rng.uniform(-0.02, 0.02, size=(200, 3))).astype(np.float32)
+    sc.pos = pos
+    g, o = pair(sc)
+    g.step()
+    o.step(1)
+    on, oc = o.neighbors()
+    assert oc.max() == 63
+    compare_step(g, o)
+
+
+def test_tiny_and_empty():
+    sc = scenes.dam_break((1, 1, 1), (0, 0, 0, 1.6, 1.6, 1.6), seed=7)
+    g, o = pair(sc)
+    g.step()
+    o.step(1)
+    compare_step(g, o)
+    e = scenes.dam_break((1, 1, 1), (0, 0, 0, 1.6, 1.6, 1.6), seed=7)
+    e.pos = e.pos[:0]
+    e.vel = e.vel[:0]
+    s = Solver.from_scene(e)
+    s.step()
+    p, v = s.get_state()
+    assert p.shape == (0, 3)
+
+
+def test_density_stats_match_oracle():
+    sc = scenes.config_cpu_32k()
+    g, o = pair(sc, flags=FLAG_DENSITY_STATS)
+    g.advance(10)
+    o.step(10)
+    st = g.density_stats()
+    err = o.fields()[2].astype(np.float64)
+    ref = (np.abs(err).mean(), np.maximum(err, 0).mean(), err.max())
+    assert st[2] == np.float32(ref[2])
+    np.testing.assert_allclose(st[:2], ref[:2], rtol=1e-5)
+
+
+def test_set_get_state_roundtrip():
+    sc = scenes.dam_break((8, 8, 8), (0, 0, 0, 1.6, 1.6, 1.6), seed=8)
+    s = Solver.from_scene(sc)
+    p, v = s.get_state()
+    assert_bitexact(p, sc.pos, "positions in")
+    s.advance(3)
+    p2, v2 = s.get_state()
+    s.set_state(p2, v2)
+    p3, v3 = s.get_state()
+    assert_bitexact(p3, p2, "positions round trip")
+    assert_bitexact(v3, v2, "velocities round trip")

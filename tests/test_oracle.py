@@ -1,0 +1,462 @@
+"""Pins the CPU oracle before it is trusted as the GPU's checker.
+
+Each oracle function is checked against something independent of its own formula:
+  * the reference's own code -- golden vectors recorded from its CPU solver and scan test
+    (tests/golden/make_ref_vectors.py) and, where the reference tree is present, the live
+    reference build in oracle/_ref;
+  * closed forms (kernel values, normalisation, gradient magnitude, curl of a rigid rotation,
+    a hand-computed two-particle lambda);
+  * brute force (cell ids, stable counting sort, neighbour lists in the reference's traversal
+    order);
+  * invariants (momentum-free delta-p, zero curl / viscosity for a uniform translation,
+    determinism across thread counts, error falling with iterations).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle_lib import Oracle, fp, ip, orc, ref, ref_available, up
+from pbf_amd import SolverParameter, scenes
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+F32 = np.float32
+
+
+def params(**over):
+    p = scenes.default_params(**over)
+    return SolverParameter.from_dict(p, 0, (16, 16, 16), 0)
+
+
+# ---- sin / jitter / clamp ------------------------------------------------------------------
+def test_sin_against_libm():
+    xs = np.concatenate([np.linspace(-50, 50, 20001), np.linspace(-3000, 3000, 20001)]).astype(F32)
+    ours = np.array([orc().orc_sin(float(x)) for x in xs], np.float64)
+    exact = np.sin(xs.astype(np.float64))
+    err = np.abs(ours - exact)
+    small = np.abs(xs) <= 50
+    assert err[small].max() < 3e-7
+    assert err.max() < 2e-6
+    assert orc().orc_sin(0.0) == 0.0
+    assert math.isnan(orc().orc_sin(float("nan")))
+    assert math.isnan(orc().orc_sin(float("inf")))
+    assert orc().orc_sin(2.0e6) == 0.0
+
+
+def test_random_offset_range_and_spread():
+    rng = np.random.default_rng(1)
+    uv = rng.uniform(0, 6.4, size=(20000, 2)).astype(F32)
+    r = np.array([orc().orc_random_offset(float(u), float(v)) for u, v in uv])
+    assert (r >= 0).all() and (r < 1).all()
+    assert abs(r.mean() - 0.5) < 0.02
+    assert np.histogram(r, bins=10, range=(0, 1))[0].min() > 1500
+
+
+@pytest.mark.parametrize("axis,side", [(0, 0), (0, 1), (1, 0), (1, 1), (2, 0), (2, 1)])
+def test_bound_faces(axis, side):
+    """BoundParticle (cuh:121-140): the clamped axis moves inside by coeff*rand(other two axes);
+    y-max draws rand from (y, z) (cuh:132), every other face from the two other coordinates."""
+    bd = np.array([0.03, 0.03, 0.03, 1.57, 1.57, 1.57], F32)
+    p = np.array([0.5, 0.7, 0.9], F32)
+    p[axis] = -0.2 if side == 0 else 1.9
+    q = p.copy()
+    orc().orc_bound(fp(q), fp(bd), 0.001)
+    others = [a for a in range(3) if a != axis]
+    uv = (p[1], p[2]) if (axis == 1 and side == 1) else (p[others[0]], p[others[1]])
+    ro = orc().orc_random_offset(float(uv[0]), float(uv[1]))
+    want = F32(bd[axis] + F32(0.001) * F32(ro)) if side == 0 else F32(bd[3 + axis] - F32(0.001) * F32(ro))
+    assert q[axis] == want
+    for a in others:
+        assert q[a] == p[a]
+    inside = np.array([0.5, 0.7, 0.9], F32)
+    q = inside.copy()
+    orc().orc_bound(fp(q), fp(bd), 0.001)
+    assert np.array_equal(q, inside)
+
+
+# ---- smoothing kernels ---------------------------------------------------------------------
+def poly6(r, P):
+    v = np.asarray(r, F32)
+    return orc().orc_poly6(P, fp(v))
+
+
+def spiky(r, P):
+    v = np.asarray(r, F32)
+    g = np.empty(3, F32)
+    orc().orc_spiky_grad(P, fp(v), fp(g))
+    return g
+
+
+def test_poly6_closed_forms():
+    P = params()
+    h = 0.1
+    assert poly6([0, 0, 0], P) == pytest.approx(315 / (64 * math.pi * h**3), rel=2e-6)
+    assert poly6([0.1000001, 0, 0], P) == 0.0
+    assert poly6([0.2, 0, 0], P) == 0.0
+    # normalisation: 4 pi int_0^h r^2 W(r) dr = 1
+    r = np.linspace(0, h, 4001)
+    w = np.array([poly6([x, 0, 0], P) for x in r], np.float64)
+    integral = np.trapezoid(4 * math.pi * r**2 * w, r)
+    assert integral == pytest.approx(1.0, abs=2e-4)
+
+
+def test_spiky_closed_forms():
+    P = params()
+    h = 0.1
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        rlen = rng.uniform(0.002, 0.0999)
+        g = spiky(d * rlen, P).astype(np.float64)
+        mag = 45 / (math.pi * h**6) * (h - rlen) ** 2
+        assert np.linalg.norm(g) == pytest.approx(mag, rel=2e-5)
+        assert np.dot(g, d) < 0  # points from i towards j: -r_hat
+        np.testing.assert_allclose(g / np.linalg.norm(g), -d, atol=2e-5)
+    assert not spiky([0.0005, 0, 0], P).any()  # below the cut-off (0.01 h)
+    assert not spiky([0.1001, 0, 0], P).any()
+
+
+@pytest.mark.skipif(not ref_available(), reason="reference build oracle/_ref not present")
+def test_kernels_against_reference_code():
+    """Wpoly6 / delWspiky of the reference's src/solver.h (its literal 0.01 cut-off)."""
+    P = params(grad_rmin=0.01)
+    rng = np.random.default_rng(4)
+    for _ in range(300):
+        r = rng.uniform(-0.06, 0.06, size=3).astype(F32)
+        w_ref = ref().ref_wpoly6(fp(r), 0.1)
+        assert poly6(r, P) == pytest.approx(w_ref, rel=2e-6, abs=1e-3)
+        g_ref = np.empty(3, F32)
+        ref().ref_spiky_grad(fp(r), 0.1, fp(g_ref))
+        np.testing.assert_allclose(spiky(r, P), g_ref, rtol=2e-5, atol=1e-2)
+
+
+# ---- scan / hash / sort ----------------------------------------------------------------------
+def test_scan_against_reference_scan_test():
+    g = np.load(os.path.join(GOLDEN, "ref_scan.npz"))
+    data = g["data"].astype(np.int32)
+    out = np.empty_like(data)
+    orc().orc_exclusive_scan(ip(data), data.size, ip(out))
+    assert np.array_equal(out, g["serial"].astype(np.int32))
+    assert np.array_equal(out, g["blelloch"].astype(np.int32))
+
+
+@pytest.mark.skipif(not ref_available(), reason="reference build oracle/_ref not present")
+def test_scan_against_live_reference_64cubed():
+    """The reference's own scan test, testPrefixSum.cpp:65-86: 64^3 counts in [0, 16)."""
+    rng = np.random.default_rng(7)
+    data = rng.integers(0, 16, size=64**3).astype(np.uint32)
+    a = np.empty_like(data)
+    b = np.empty_like(data)
+    ref().ref_scan_serial(up(data), up(a), data.size)
+    ref().ref_scan_blelloch(up(data), up(b), data.size)
+    d32 = data.astype(np.int32)
+    out = np.empty_like(d32)
+    orc().orc_exclusive_scan(ip(d32), d32.size, ip(out))
+    assert np.array_equal(out, a.astype(np.int32))
+    assert np.array_equal(out, b.astype(np.int32))
+
+
+def brute_cells(pos, gmin, h, dims):
+    invH = F32(1.0) / F32(h)
+    out = np.empty(len(pos), np.int64)
+    for i, p in enumerate(pos):
+        d = []
+        for a in range(3):
+            f = (F32(p[a]) - F32(gmin[a])) * invH
+            d.append(None if np.isnan(f) else int(math.trunc(float(f))) if abs(float(f)) < 1e9 else -(10**9))
+        ok = all(v is not None and 0 <= v < dims[a] for a, v in enumerate(d))
+        out[i] = d[0] + d[1] * dims[0] + d[2] * dims[0] * dims[1] if ok else -1
+    return out
+
+
+def test_cell_ids_brute_force_and_edges():
+    dims = np.array([16, 8, 12], np.int32)
+    gmin = np.array([0.0, -0.4, 0.2], F32)
+    rng = np.random.default_rng(5)
+    pos = rng.uniform(-0.3, 1.9, size=(3000, 3)).astype(F32)
+    edges = np.array(
+        [
+            [-0.05, 0.0, 0.5],  # (-h, 0) truncates to cell 0 (cuh:157)
+            [-0.15, 0.0, 0.5],  # -1.5 h -> -1: outside
+            [1.6, 0.0, 0.5],  # exactly dims*h: outside
+            [1.5999999, 0.0, 0.5],
+            [np.nan, 0.0, 0.5],
+            [1e30, 0.0, 0.5],
+            [0.0, -0.4, 0.2],
+        ],
+        F32,
+    )
+    pos = np.concatenate([pos, edges])
+    gid = np.empty(len(pos), np.int32)
+    orc().orc_cell_ids(fp(pos), len(pos), fp(gmin), 0.1, ip(dims), ip(gid))
+    assert np.array_equal(gid, brute_cells(pos, gmin, 0.1, dims))
+    assert gid[-7] >= 0 and gid[-6] == -1 and gid[-5] == -1 and gid[-3] == -1 and gid[-2] == -1 and gid[-1] == 0
+
+
+def small_scene(seed=9, side=(12, 12, 12), **over):
+    sc = scenes.dam_break(side, (0, 0, 0, 1.6, 1.6, 1.6), seed=seed)
+    sc.params.update(over)
+    return sc
+
+
+def test_sort_is_stable_counting_sort():
+    sc = small_scene()
+    o = Oracle.from_scene(sc)
+    o.step(3)
+    gid, cnt, cs, srt, nin = o.grid()
+    inside = gid >= 0
+    assert nin == inside.sum()
+    expect = np.argsort(np.where(inside, gid, np.iinfo(np.int32).max), kind="stable")[:nin]
+    assert np.array_equal(srt[:nin], expect)
+    assert np.array_equal(cnt, np.bincount(gid[inside], minlength=o.ncells))
+    assert cs[0] == 0 and np.array_equal(cs[1:], np.cumsum(cnt))
+
+
+def brute_neighbors(pos, gid_dims, gmin, h, cap):
+    """Neighbour lists in the reference's order (cu:133-152): cells z-outer/y/x-inner, members by
+    ascending id, self excluded, r.r <= h*h in fp32, at most cap."""
+    n = len(pos)
+    invH = F32(1.0) / F32(h)
+    dims = gid_dims
+    d = np.trunc((pos - gmin) * invH).astype(np.int64)
+    cells = {}
+    for i in range(n):
+        if all(0 <= d[i, a] < dims[a] for a in range(3)):
+            cells.setdefault(tuple(d[i]), []).append(i)
+    h2 = F32(h) * F32(h)
+    out = []
+    for i in range(n):
+        lst = []
+        for j in range(27):
+            c = (d[i, 0] + j % 3 - 1, d[i, 1] + (j // 3) % 3 - 1, d[i, 2] + j // 9 - 1)
+            for k in cells.get(c, []):
+                if k == i or len(lst) >= cap:
+                    continue
+                r = pos[i] - pos[k]
+                r2 = F32(F32(r[0] * r[0]) + F32(r[1] * r[1])) + F32(r[2] * r[2])
+                if r2 <= h2:
+                    lst.append(k)
+        out.append(lst)
+    return out
+
+
+def test_neighbors_brute_force_order_and_cap():
+    sc = lattice_scene((10, 10, 10), jitter=0.005)
+    pos = sc.pos.copy()
+    rng = np.random.default_rng(11)
+    pos[:120] = (np.array([0.8, 0.8, 0.8]) + rng.uniform(-0.03, 0.03, size=(120, 3))).astype(F32)  # exceeds cap
+    sc.pos = pos
+    o = Oracle.from_scene(sc, iterations=0)
+    o.step(1)
+    p_pred, _ = o.get_state()  # zero gravity, zero velocity: predicted == initial (no clamping here)
+    assert np.array_equal(p_pred, pos)
+    nbr, cnt = o.neighbors()
+    want = brute_neighbors(pos, sc.grid_dims, sc.box[:3].astype(F32), 0.1, 63)
+    assert max(len(w) for w in want) == 63
+    for i in range(len(pos)):
+        assert list(nbr[i, : cnt[i]]) == want[i]
+
+
+# ---- against the reference's CPU solver ------------------------------------------------------
+def run_probe_with_oracle(g):
+    pos, box = g["pos"], g["box"]
+    prm = g["params"]
+    p = scenes.default_params(
+        h=float(prm[0]),
+        restRho=float(prm[1]),
+        epsilon=float(prm[2]),
+        k=float(prm[3]),
+        n=float(prm[4]),
+        deltaq=float(prm[5]),
+        radius=float(prm[6]),
+        gravity=(0.0, 0.0, 0.0),
+        iterations=1,
+        grad_rmin=0.01,
+        c=0.0,
+        vorticityConfinementEpilon=0.0,
+    )
+    para = SolverParameter.from_dict(p, len(pos), (64, 64, 64), 0)
+    bd = np.concatenate([box[:3] + F32(prm[6]), box[3:] - F32(prm[6])]).astype(F32)
+    o = Oracle(para, box, bd, pos)
+    o.step(1)
+    return o
+
+
+def test_oracle_against_reference_golden_vectors():
+    """Grid, sorted members, neighbour sets, lambda and delta-p of the reference's CPU solver on
+    the same input (recorded in tests/golden/ref_probe_3k.npz)."""
+    g = np.load(os.path.join(GOLDEN, "ref_probe_3k.npz"))
+    o = run_probe_with_oracle(g)
+    gid, cnt, cs, srt, nin = o.grid()
+    assert nin == int(g["n_in_grid"])
+    assert np.array_equal(gid, g["gid"])
+    cells = g["cells"]
+    assert np.array_equal(cs[cells], g["cell_start"])
+    assert np.array_equal(cs[cells + 1], g["cell_end"])
+    # inside a cell the reference's CPU scatter gives descending ids (solver.cpp:23-29)
+    for c, a, b in zip(cells, g["cell_start"], g["cell_end"]):
+        assert list(srt[a:b]) == sorted(g["sorted_ids"][a:b])
+        assert list(g["sorted_ids"][a:b]) == sorted(g["sorted_ids"][a:b], reverse=True)
+    nbr, ncnt = o.neighbors()
+    rn = g["nbr"]
+    rc = rn[:, 63]
+    assert ncnt.max() < 63
+    assert np.array_equal(ncnt, rc)
+    for i in range(len(ncnt)):
+        assert set(nbr[i, : ncnt[i]]) == set(rn[i, : rc[i]])
+    lam = o.fields()[0]
+    scale = np.abs(g["lam"]).max()
+    np.testing.assert_allclose(lam, g["lam"], rtol=0, atol=2e-5 * scale)
+    dp = o.dp()
+    scale = np.abs(g["offset"]).max()
+    np.testing.assert_allclose(dp, g["offset"], rtol=0, atol=5e-5 * scale)
+
+
+@pytest.mark.skipif(not ref_available(), reason="reference build oracle/_ref not present")
+def test_golden_vectors_are_reproducible():
+    """The committed vectors are what the live reference build produces."""
+    import ctypes as C  # noqa: F401
+
+    g = np.load(os.path.join(GOLDEN, "ref_probe_3k.npz"))
+    n = len(g["pos"])
+    cells = ref().ref_grid_cells()
+    gid = np.empty(n, np.int32)
+    cs = np.empty(cells, np.int32)
+    ce = np.empty(cells, np.int32)
+    srt = np.empty(n, np.int32)
+    nbr = np.empty((n, 64), np.int32)
+    lam = np.empty(n, F32)
+    off = np.empty((n, 3), F32)
+    ref().ref_probe(fp(g["params"]), fp(g["box"]), fp(g["pos"]), n, ip(gid), ip(cs), ip(ce), ip(srt), ip(nbr), fp(lam), fp(off))
+    assert np.array_equal(gid, g["gid"])
+    cnt = nbr[:, 63].copy()
+    assert np.array_equal(cnt, g["nbr"][:, 63])
+    for i in range(n):
+        assert np.array_equal(nbr[i, : cnt[i]], g["nbr"][i, : cnt[i]])
+    assert np.array_equal(lam, g["lam"])
+
+
+# ---- closed forms and invariants of the step --------------------------------------------------
+def two_particle_case(r):
+    pos = np.array([[0.8, 0.8, 0.8], [0.8 + r, 0.8, 0.8]], F32)
+    sc = small_scene()
+    sc.pos = pos
+    sc.vel = np.zeros_like(pos)
+    o = Oracle.from_scene(sc, gravity=(0.0, 0.0, 0.0), iterations=1, c=0.0, vorticityConfinementEpilon=0.0)
+    o.step(1)
+    return o, sc.params
+
+
+def test_lambda_two_particles_hand_computed():
+    r = 0.043
+    o, p = two_particle_case(r)
+    h, rho0, eps = 0.1, p["restRho"], p["epsilon"]
+    w = 315 / (64 * math.pi * h**9) * (h * h - r * r) ** 3
+    g = 45 / (math.pi * h**6) * (h - r) ** 2  # |grad W|
+    lam = -(w / rho0 - 1) / ((g * g + g * g) / rho0**2 + eps)
+    lf = o.fields()[0]
+    assert lf[0] == pytest.approx(lam, rel=2e-5)
+    assert lf[1] == lf[0]
+    # delta-p of particle 0 points away from / towards particle 1 by the sign of lambda + s_corr
+    k, n, dq = p["k"], p["n"], p["deltaq"]
+    wdq = 315 / (64 * math.pi * h**9) * (h * h - dq * dq) ** 3
+    s = -k * (w / wdq) ** n
+    dp0 = (lam + lam + s) * (-g) / rho0  # x component: grad W(x0 - x1) = -|g| * (-x_hat)... along -r_hat
+    dp = o.dp()
+    assert dp[0, 0] == pytest.approx(-dp0, rel=5e-5)
+    assert dp[1, 0] == pytest.approx(dp0, rel=5e-5)
+
+
+def lattice_scene(side, spacing=0.05, jitter=0.0, **over):
+    pos = scenes.lattice(side, spacing, (0.4, 0.4, 0.4), seed=1, jitter=jitter)
+    sc = scenes.dam_break((1, 1, 1), (0, 0, 0, 1.6, 1.6, 1.6))
+    sc.pos = pos
+    sc.vel = np.zeros_like(pos)
+    sc.params.update(dict(gravity=(0.0, 0.0, 0.0)))
+    sc.params.update(over)
+    return sc
+
+
+def test_delta_p_sums_to_zero():
+    """(lambda_i + lambda_j + s_corr) is symmetric and grad W antisymmetric: sum delta-p = 0."""
+    sc = lattice_scene((9, 9, 9), jitter=0.008, iterations=1)
+    o = Oracle.from_scene(sc)
+    o.step(1)
+    nbr, cnt = o.neighbors()
+    assert cnt.max() < 63
+    dp = o.dp().astype(np.float64)
+    assert np.abs(dp.sum(0)).max() < 1e-5 * np.abs(dp).sum(0).max()
+
+
+def test_translation_has_no_curl_or_viscosity():
+    sc = lattice_scene((8, 8, 8), jitter=0.005, iterations=0, c=1e-3, vorticityConfinementEpilon=1e-3)
+    sc.vel[:] = np.array([0.3, -0.2, 0.1], F32)
+    o = Oracle.from_scene(sc)
+    o.step(1)
+    p, v = o.get_state()
+    assert not o.fields()[3].any()
+    assert np.array_equal(v, sc.vel)
+
+
+def test_curl_of_rigid_rotation():
+    """v = Omega x x on a jitter-free lattice: for an interior particle the reference's curl
+    sum_j (v_j - v_i) x gradW(x_i - x_j) equals (2/3) Omega sum_j f(|r_j|) |r_j| with
+    gradW = f(|r|) r_hat, f = -45/(pi h^6) (h - |r|)^2 (lattice isotropy)."""
+    sc = lattice_scene((9, 9, 9), jitter=0.0, iterations=0)
+    omega = np.array([0.2, -0.5, 0.9])
+    sc.vel = np.cross(omega, sc.pos.astype(np.float64)).astype(F32)
+    o = Oracle.from_scene(sc)
+    o.step(1)
+    curl = o.fields()[3].astype(np.float64)
+    i = 4 + 9 * 4 + 81 * 4  # centre particle
+    h = 0.1
+    rs = []
+    for dx in range(-2, 3):
+        for dy in range(-2, 3):
+            for dz in range(-2, 3):
+                r = 0.05 * math.sqrt(dx * dx + dy * dy + dz * dz)
+                if 0 < r <= h:
+                    rs.append(r)
+    s = sum(-45 / (math.pi * h**6) * (h - r) ** 2 * r for r in rs)
+    np.testing.assert_allclose(curl[i], (2.0 / 3.0) * omega * s, rtol=2e-3, atol=1e-3 * abs(s))
+
+
+def test_iterations_reduce_interior_density_error():
+    """The Jacobi loop is not monotone per iteration (the reference has no relaxation), but many
+    iterations leave interior particles far closer to the rest density than none."""
+    sc = lattice_scene((10, 10, 10), spacing=0.045, jitter=0.004)
+    errs = {}
+    for it in (0, 8, 16):
+        o = Oracle.from_scene(sc, iterations=it)
+        o.step(1)
+        e = o.fields()[2]
+        nb, nc = o.neighbors()
+        errs[it] = np.abs(e[nc >= 25]).mean()
+    assert errs[8] < 0.75 * errs[0] and errs[16] < 0.75 * errs[0]
+
+
+def test_deterministic_across_thread_counts():
+    sc = scenes.config_cpu_32k()
+    outs = []
+    for t in (1, 8):
+        orc().orc_set_threads(t)
+        o = Oracle.from_scene(sc)
+        o.step(5)
+        outs.append(o.get_state())
+    orc().orc_set_threads(os.cpu_count() or 1)
+    assert np.array_equal(outs[0][0].view(np.uint32), outs[1][0].view(np.uint32))
+    assert np.array_equal(outs[0][1].view(np.uint32), outs[1][1].view(np.uint32))
+
+
+def test_dam_break_stays_bounded():
+    """The reference-unit parameter mapping keeps the 32k dam break physical (the literal
+    physical constants do not: see pbf_amd.scenes)."""
+    sc = scenes.config_cpu_32k()
+    o = Oracle.from_scene(sc)
+    o.step(60)
+    p, v = o.get_state()
+    err = o.fields()[2]
+    assert np.isfinite(p).all() and np.abs(v).max() < 30.0
+    assert np.abs(err).mean() < 0.15
