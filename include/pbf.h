@@ -160,6 +160,67 @@ int pbf_stage_times(pbf_solver* s, float out[7]);
  * vort_apply).  Returns the number of groups written (<= max) or an error. */
 int pbf_profile_step(pbf_solver* s, float* ms, const char** names, int32_t max);
 
+/* ---- learned warm start and adaptive iteration (no counterpart in the reference) ----------
+ * A per-particle MLP 16 -> 64 -> 64 -> 64 -> 1 (SiLU), BF16 weights and activations, FP32
+ * accumulation on MFMA, predicts the multiplier a plain step accumulates over its Jacobi
+ * iterations (divided by h^2) from 16 features of the particle at its predicted position:
+ *   0 C = rho/restRho - 1   1 neighbour count   2 sum_j |grad_j C|^2   3 |grad_i C|
+ *   4-6 v dt / h   7 previous step's multiplier / h^2   8 distance to the nearest wall / h
+ *   9-14 distance to each wall (x-, x+, y-, y+, z-, z+) / h   15 plain iteration-1 lambda / h^2
+ * Parameters are packed FP32: W1[64][16] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64] b4. */
+typedef struct pbf_mlp pbf_mlp;
+int pbf_mlp_param_count(void);
+/* New network with seeded He-uniform weights on the current device. */
+int pbf_mlp_create(uint64_t seed, pbf_mlp** out);
+int pbf_mlp_destroy(pbf_mlp* m);
+int pbf_mlp_get_params(pbf_mlp* m, float* params);
+int pbf_mlp_set_params(pbf_mlp* m, const float* params);
+/* Input standardisation x' = clip((x - mean) * inv_std, -8, 8); output multiplier / h^2 =
+ * y * yscale + yshift. */
+int pbf_mlp_set_normalization(pbf_mlp* m, const float* feat_mean, const float* feat_inv_std, float yscale,
+                              float yshift);
+/* y[n] (the network output before yscale / yshift) for raw features x[n][16] (host arrays). */
+int pbf_mlp_forward(pbf_mlp* m, const float* x, int32_t n, float* y);
+/* Mean squared error of y against t over n samples and its gradient (host arrays; grad may be
+ * NULL). */
+int pbf_mlp_loss_grad(pbf_mlp* m, const float* x, const float* t, int32_t n, float* grad, float* loss);
+/* One Adam step with grad (host) or, when NULL, the gradient of the last pbf_mlp_loss_grad. */
+int pbf_mlp_adam_step(pbf_mlp* m, const float* grad, float lr, float beta1, float beta2, float eps);
+/* steps Adam steps on random batches of a device-resident dataset (dev_x [nsamples][16] raw
+ * features, dev_t [nsamples] normalised targets), learning rate cosine-decayed from lr to
+ * lr_final; losses[steps] (host, may be NULL) receives the batch losses. */
+int pbf_mlp_train(pbf_mlp* m, const float* dev_x, const float* dev_t, int64_t nsamples, int32_t batch, int32_t steps,
+                  float lr, float lr_final, uint64_t seed, float* losses);
+/* Use m for the warm start of s (NULL detaches); m must outlive its use. */
+int pbf_attach_mlp(pbf_solver* s, pbf_mlp* m);
+
+/* One plain step with `iterations` iterations that also records, for each particle whose id
+ * satisfies id % stride == phase, its 16 raw features and accumulated multiplier / h^2 at row
+ * base + id / stride of the device arrays dev_x [cap][16] and dev_t [cap]. */
+int pbf_record_step(pbf_solver* s, int32_t iterations, float* dev_x, float* dev_t, int64_t base, int64_t cap,
+                    int32_t stride, int32_t phase);
+
+/* One step whose constraint loop stops once the compression residual (mean over particles of
+ * max(C_i, 0), C_i = rho_i/restRho - 1 over the frozen lists) is <= tol after at least
+ * min_iters iterations, or after max_iters.  use_mlp: iteration 1 starts from the attached
+ * network's multiplier instead of lambda.  The guess is kept if its residual is <= tol or below
+ * the predicted state's, and plain iterations continue from it (the fallback that judges its
+ * error); otherwise it is rejected and the step runs plain from the predicted state, exactly as
+ * with use_mlp = 0.  Writes the iterations run (the guess counts as one) and the last residual
+ * measured (either pointer may be NULL).  Launched eagerly (one residual read per iteration). */
+int pbf_step_adaptive(pbf_solver* s, int32_t use_mlp, float tol, int32_t min_iters, int32_t max_iters,
+                      int32_t* iters_used, float* resid);
+
+/* Of the last use_mlp step: out[0] = residual of the predicted state, out[1] = residual after the
+ * guess, out[2] = 1 if the guess was kept, 0 if it was rejected. */
+int pbf_last_guess_info(pbf_solver* s, float out[3]);
+
+/* Device memory helpers for callers without an allocator (datasets, export targets). */
+int pbf_device_alloc(uint64_t bytes, void** out);
+int pbf_device_free(void* p);
+int pbf_memcpy_to_host(void* dst, const void* src, uint64_t bytes);
+int pbf_memcpy_to_device(void* dst, const void* src, uint64_t bytes);
+
 /* Library build string. */
 const char* pbf_version(void);
 

@@ -150,6 +150,24 @@ def lib() -> C.CDLL:
         "pbf_stage_times": (C.c_int, [_VP, _F]),
         "pbf_version": (C.c_char_p, []),
         "pbf_profile_step": (C.c_int, [_VP, _F, C.POINTER(C.c_char_p), C.c_int32]),
+        "pbf_mlp_param_count": (C.c_int, []),
+        "pbf_mlp_create": (C.c_int, [C.c_uint64, C.POINTER(_VP)]),
+        "pbf_mlp_destroy": (C.c_int, [_VP]),
+        "pbf_mlp_get_params": (C.c_int, [_VP, _F]),
+        "pbf_mlp_set_params": (C.c_int, [_VP, _F]),
+        "pbf_mlp_set_normalization": (C.c_int, [_VP, _F, _F, C.c_float, C.c_float]),
+        "pbf_mlp_forward": (C.c_int, [_VP, _F, C.c_int32, _F]),
+        "pbf_mlp_loss_grad": (C.c_int, [_VP, _F, _F, C.c_int32, _F, _F]),
+        "pbf_mlp_adam_step": (C.c_int, [_VP, _F, C.c_float, C.c_float, C.c_float, C.c_float]),
+        "pbf_mlp_train": (C.c_int, [_VP, _VP, _VP, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_uint64, _F]),
+        "pbf_attach_mlp": (C.c_int, [_VP, _VP]),
+        "pbf_record_step": (C.c_int, [_VP, C.c_int32, _VP, _VP, C.c_int64, C.c_int64, C.c_int32, C.c_int32]),
+        "pbf_step_adaptive": (C.c_int, [_VP, C.c_int32, C.c_float, C.c_int32, C.c_int32, _I, _F]),
+        "pbf_last_guess_info": (C.c_int, [_VP, _F]),
+        "pbf_device_alloc": (C.c_int, [C.c_uint64, C.POINTER(_VP)]),
+        "pbf_device_free": (C.c_int, [_VP]),
+        "pbf_memcpy_to_host": (C.c_int, [_VP, _VP, C.c_uint64]),
+        "pbf_memcpy_to_device": (C.c_int, [_VP, _VP, C.c_uint64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -302,10 +320,145 @@ class Solver:
         cnt = _check(lib().pbf_profile_step(self._h, _fp(ms), names, 64))
         return [(names[i].decode(), float(ms[i])) for i in range(cnt)]
 
+    # warm start / adaptive iteration ------------------------------------------------------
+    def attach_mlp(self, mlp: "Mlp | None"):
+        _check(lib().pbf_attach_mlp(self._h, None if mlp is None else mlp._h))
+        self._mlp = mlp
+
+    def step_adaptive(self, use_mlp: bool, tol: float, min_iters: int, max_iters: int):
+        """One step stopping at compression residual <= tol; returns (iterations, residual)."""
+        it = np.zeros(1, np.int32)
+        r = np.zeros(1, np.float32)
+        _check(lib().pbf_step_adaptive(self._h, int(bool(use_mlp)), float(tol), int(min_iters), int(max_iters), _ip(it), _fp(r)))
+        return int(it[0]), float(r[0])
+
+    def last_guess_info(self):
+        """(residual of the predicted state, residual after the guess, guess kept) of the last learned step."""
+        out = np.zeros(3, np.float32)
+        _check(lib().pbf_last_guess_info(self._h, _fp(out)))
+        return float(out[0]), float(out[1]), bool(out[2])
+
+    def record_step(self, iterations: int, ds: "DeviceDataset", base: int, stride: int, phase: int):
+        _check(lib().pbf_record_step(self._h, int(iterations), ds.x, ds.t, int(base), int(ds.cap), int(stride), int(phase)))
+
     def stage_times(self):
         out = np.zeros(7, np.float32)
         _check(lib().pbf_stage_times(self._h, _fp(out)))
         return out
+
+
+class DeviceBuffer:
+    """Device allocation owned from Python (datasets, export targets)."""
+
+    def __init__(self, nbytes: int):
+        h = _VP()
+        _check(lib().pbf_device_alloc(int(nbytes), C.byref(h)))
+        self.ptr = h.value
+        self.nbytes = int(nbytes)
+
+    def to_host(self, dtype, count):
+        out = np.empty(count, dtype)
+        _check(lib().pbf_memcpy_to_host(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def from_host(self, arr):
+        arr = np.ascontiguousarray(arr)
+        _check(lib().pbf_memcpy_to_device(self.ptr, arr.ctypes.data, arr.nbytes))
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().pbf_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceDataset:
+    """(features [cap][16], targets [cap]) in device memory, filled by Solver.record_step."""
+
+    def __init__(self, cap: int):
+        self.cap = int(cap)
+        self._x = DeviceBuffer(self.cap * 16 * 4)
+        self._t = DeviceBuffer(self.cap * 4)
+        self.x = self._x.ptr
+        self.t = self._t.ptr
+
+    def host(self, count=None):
+        count = self.cap if count is None else int(count)
+        x = self._x.to_host(np.float32, self.cap * 16).reshape(self.cap, 16)[:count]
+        t = self._t.to_host(np.float32, self.cap)[:count]
+        return x, t
+
+    def upload(self, x, t):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 16)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1)
+        assert x.shape[0] <= self.cap and t.shape[0] == x.shape[0]
+        self._x.from_host(x)
+        self._t.from_host(t)
+
+    def close(self):
+        self._x.close()
+        self._t.close()
+
+
+class Mlp:
+    """The warm-start network (pbf_mlp): 16 -> 64 -> 64 -> 64 -> 1, BF16 MFMA kernels."""
+
+    def __init__(self, seed: int = 0):
+        h = _VP()
+        _check(lib().pbf_mlp_create(int(seed), C.byref(h)))
+        self._h = h
+        self.nparams = lib().pbf_mlp_param_count()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pbf_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.empty(self.nparams, np.float32)
+        _check(lib().pbf_mlp_get_params(self._h, _fp(p)))
+        return p
+
+    def set_params(self, p):
+        p = _f32(p, (self.nparams,))
+        _check(lib().pbf_mlp_set_params(self._h, _fp(p)))
+
+    def set_normalization(self, mean, inv_std, yscale=1.0, yshift=0.0):
+        _check(lib().pbf_mlp_set_normalization(self._h, _fp(_f32(mean, (16,))), _fp(_f32(inv_std, (16,))), float(yscale), float(yshift)))
+
+    def forward(self, x):
+        x = _f32(x, (-1, 16))
+        y = np.empty(x.shape[0], np.float32)
+        _check(lib().pbf_mlp_forward(self._h, _fp(x), x.shape[0], _fp(y)))
+        return y
+
+    def loss_grad(self, x, t):
+        x = _f32(x, (-1, 16))
+        t = _f32(t, (x.shape[0],))
+        g = np.empty(self.nparams, np.float32)
+        loss = np.zeros(1, np.float32)
+        _check(lib().pbf_mlp_loss_grad(self._h, _fp(x), _fp(t), x.shape[0], _fp(g), _fp(loss)))
+        return float(loss[0]), g
+
+    def adam_step(self, grad=None, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        g = None if grad is None else _f32(grad, (self.nparams,))
+        _check(lib().pbf_mlp_adam_step(self._h, _fp(g), float(lr), float(beta1), float(beta2), float(eps)))
+
+    def train(self, ds: DeviceDataset, nsamples: int, batch: int, steps: int, lr=2e-3, lr_final=1e-4, seed=0):
+        losses = np.zeros(max(steps, 1), np.float32)
+        _check(lib().pbf_mlp_train(self._h, ds.x, ds.t, int(nsamples), int(batch), int(steps), float(lr), float(lr_final), int(seed), _fp(losses)))
+        return losses[:steps]
 
 
 # ---- the reference's host API names (kernels/cudaPBF.h:8-12) ------------------------------

@@ -19,7 +19,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpbf_hip.so")
 
-SOURCES = ["pbf_capi.hip", "pbf_kernels.hip"]
+SOURCES = ["pbf_capi.hip", "pbf_kernels.hip", "pbf_learned.hip"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950",

@@ -11,12 +11,14 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "pbf.h"
 #include "pbf_kernels.h"
+#include "pbf_learned.h"
 
 using namespace pbf;
 
@@ -66,6 +68,15 @@ struct pbf_solver {
     float* partial = nullptr;
     int npartial = 0;
     bool stats_valid = false;
+    // warm start: features, accumulated multiplier, residual partials, attached network
+    float* feat = nullptr;
+    float* lam_sum = nullptr;
+    float* resid = nullptr;
+    float* resid0 = nullptr;
+    float* resid_host = nullptr;
+    float last_pred_resid = 0.0f, last_guess_resid = 0.0f;
+    int guess_accepted = 0;
+    const pbf::MlpModel* mlp = nullptr;
     // per-launch profiling: when set, enqueue_step records an event after every launch
     std::vector<hipEvent_t>* prof_events = nullptr;
     std::vector<const char*>* prof_names = nullptr;
@@ -143,6 +154,8 @@ int dalloc(pbf_solver* s, T** p, size_t count) {
 }
 
 void free_all(pbf_solver* s) {
+    std::free(s->resid_host);
+    s->resid_host = nullptr;
     for (int i = 0; i < 4; ++i)
         if (s->graph[i]) {
             (void)hipGraphExecDestroy(s->graph[i]);
@@ -199,16 +212,17 @@ int enqueue_step(pbf_solver* s, hipEvent_t* ev) {
     if (ev) HIP_TRY(hipEventRecord(ev[3], st));
     int a = s->pc ^ 1;
     int v = s->vc ^ 1;
-    launch_neighbors_lambda(st, K, B.pos[a], B, dbg ? s->dbg_lam : nullptr);
+    const LearnArgs L0{};
+    launch_neighbors_lambda(st, K, B.pos[a], B, dbg ? s->dbg_lam : nullptr, L0);
     PROF("neighbors_lambda");
     if (ev) HIP_TRY(hipEventRecord(ev[4], st));
     const int I = s->P.iterations;
     for (int it = 0; it < I; ++it) {
         if (it > 0) {
-            launch_lambda(st, K, B.pos[a], B);
+            launch_lambda(st, K, B.pos[a], B, L0);
             PROF("lambda");
         }
-        launch_delta_apply(st, K, B.pos[a], B.pos[a ^ 1], B.vel[v], B, it == I - 1);
+        launch_delta_apply(st, K, B.pos[a], B.pos[a ^ 1], B.vel[v], B, it == I - 1, it == 0, L0);
         PROF("delta_apply");
         a ^= 1;
     }
@@ -216,7 +230,7 @@ int enqueue_step(pbf_solver* s, hipEvent_t* ev) {
     launch_curl(st, K, B.pos[a], B.vel[v], B, dbg ? s->dbg_err : nullptr, stats ? s->partial : nullptr);
     PROF("curl");
     if (ev) HIP_TRY(hipEventRecord(ev[6], st));
-    launch_vort_apply(st, K, B.pos[a], B.vel[v], B.vel[v ^ 1], B);
+    launch_vort_apply(st, K, B.pos[a], B.vel[v], B.vel[v ^ 1], B, nullptr);
     PROF("vort_apply");
     if (ev) HIP_TRY(hipEventRecord(ev[7], st));
     HIP_TRY(hipGetLastError());
@@ -323,6 +337,11 @@ int pbf_create(const pbf_params* para, const float grid_space[6], const float bo
         (rc = dalloc(s, &B.nbr, (size_t)s->cap * s->nstride)) || (rc = dalloc(s, &B.ncnt, np + 16)))
         return fail(rc);
     s->npartial = err_partial_blocks((int)np);
+    if ((rc = dalloc(s, &s->feat, np * kNumFeatures)) || (rc = dalloc(s, &s->lam_sum, np)) ||
+        (rc = dalloc(s, &s->resid, (size_t)s->npartial)) || (rc = dalloc(s, &s->resid0, (size_t)s->npartial)))
+        return fail(rc);
+    s->resid_host = (float*)std::malloc(sizeof(float) * s->npartial);
+    if (!s->resid_host) return fail(set_error(PBF_ERR_ARG, "host allocation failed"));
     if ((rc = dalloc(s, &s->dbg_gid, np)) || (rc = dalloc(s, &s->dbg_lam, np)) || (rc = dalloc(s, &s->dbg_err, np)) ||
         (rc = dalloc(s, &s->partial, (size_t)s->npartial * 3)))
         return fail(rc);
@@ -603,6 +622,424 @@ int pbf_profile_step(pbf_solver* s, float* ms, const char** names, int32_t max) 
     }
     for (hipEvent_t e : evs) (void)hipEventDestroy(e);
     return rc == PBF_OK ? cnt : rc;
+}
+
+}  // extern "C"
+
+// ============================================================================================
+// Learned warm start and adaptive stepping (no counterpart in the reference).
+
+struct pbf_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    float* p = nullptr;        // fp32 master parameters
+    float* m = nullptr;        // Adam first moment
+    float* v = nullptr;        // Adam second moment
+    float* g = nullptr;        // gradient (+ sum of squared errors at [kMlpParams])
+    pbf::MlpModel* M = nullptr;
+    float* partial = nullptr;  // per-workgroup gradient sums
+    uint32_t* idx = nullptr;
+    int idx_cap = 0;
+    int adam_t = 0;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+template <class T>
+int malloc_mlp(pbf_mlp* m, T** p, size_t count) {
+    void* q = nullptr;
+    HIP_TRY(hipMalloc(&q, std::max<size_t>(count * sizeof(T), 256)));
+    m->allocs.push_back(q);
+    *p = (T*)q;
+    return PBF_OK;
+}
+
+void free_mlp(pbf_mlp* m) {
+    for (void* q : m->allocs) (void)hipFree(q);
+    m->allocs.clear();
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    m->stream = nullptr;
+}
+
+int mlp_ensure_idx(pbf_mlp* m, int n) {
+    if (n <= m->idx_cap) return PBF_OK;
+    void* q = nullptr;
+    HIP_TRY(hipMalloc(&q, sizeof(uint32_t) * (size_t)n));
+    if (m->idx) {
+        for (auto& a : m->allocs)
+            if (a == m->idx) a = q;
+        (void)hipFree(m->idx);
+    } else {
+        m->allocs.push_back(q);
+    }
+    m->idx = (uint32_t*)q;
+    m->idx_cap = n;
+    return PBF_OK;
+}
+
+uint64_t sm64(uint64_t& st) {
+    uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Sum of the residual block partials / n (compression residual, mean max(C, 0)).
+int read_residual(pbf_solver* s, const float* partials, float* out) {
+    int nb = err_partial_blocks(std::max(s->n, 1));
+    HIP_TRY(hipMemcpyAsync(s->resid_host, partials, sizeof(float) * nb, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    double acc = 0.0;
+    for (int b = 0; b < nb; ++b) acc += s->resid_host[b];
+    *out = (float)(acc / std::max(s->n, 1));
+    return PBF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pbf_mlp_param_count(void) { return pbf::kMlpParams; }
+
+int pbf_mlp_create(uint64_t seed, pbf_mlp** out) {
+    if (!out) return set_error(PBF_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(PBF_ERR_NOGPU, "no HIP device");
+    pbf_mlp* m = new pbf_mlp();
+    (void)hipGetDevice(&m->device);
+    auto fail = [&](int rc) {
+        free_mlp(m);
+        delete m;
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_error(PBF_ERR_HIP, "hipStreamCreate failed"));
+    int rc;
+    if ((rc = malloc_mlp(m, &m->p, kMlpParams)) || (rc = malloc_mlp(m, &m->m, kMlpParams)) ||
+        (rc = malloc_mlp(m, &m->v, kMlpParams)) || (rc = malloc_mlp(m, &m->g, kMlpParams + 1)) ||
+        (rc = malloc_mlp(m, &m->M, 1)) || (rc = malloc_mlp(m, &m->partial, (size_t)256 * (kMlpParams + 1))))
+        return fail(rc);
+    // He-uniform weights (fan-in), zero biases, small output layer.
+    std::vector<float> p(kMlpParams, 0.0f);
+    uint64_t st = seed;
+    auto uni = [&](float a) { return (float)(((sm64(st) >> 40) * (1.0 / 16777216.0)) * 2.0 - 1.0) * a; };
+    for (int i = 0; i < kHid * kIn; ++i) p[kOffW1 + i] = uni(std::sqrt(6.0f / kIn));
+    for (int i = 0; i < kHid * kHid; ++i) p[kOffW2 + i] = uni(std::sqrt(6.0f / kHid));
+    for (int i = 0; i < kHid * kHid; ++i) p[kOffW3 + i] = uni(std::sqrt(6.0f / kHid));
+    for (int i = 0; i < kHid; ++i) p[kOffW4 + i] = uni(0.1f * std::sqrt(6.0f / kHid));
+    std::vector<float> zeros(kMlpParams, 0.0f);
+    if (hipMemcpy(m->p, p.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(m->m, zeros.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(m->v, zeros.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(set_error(PBF_ERR_HIP, "parameter upload failed"));
+    // identity normalisation until set
+    float fm[kIn], fi[kIn];
+    for (int i = 0; i < kIn; ++i) {
+        fm[i] = 0.0f;
+        fi[i] = 1.0f;
+    }
+    float ys[2] = {1.0f, 0.0f};
+    if (hipMemcpy(m->M->fmean, fm, sizeof(fm), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(m->M->finv, fi, sizeof(fi), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(&m->M->yscale, ys, sizeof(ys), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(set_error(PBF_ERR_HIP, "normalisation upload failed"));
+    launch_mlp_pack(m->stream, m->p, m->M);
+    if (hipStreamSynchronize(m->stream) != hipSuccess) return fail(set_error(PBF_ERR_HIP, "pack failed"));
+    *out = m;
+    return PBF_OK;
+}
+
+int pbf_mlp_destroy(pbf_mlp* m) {
+    if (!m) return PBF_OK;
+    (void)hipSetDevice(m->device);
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    free_mlp(m);
+    delete m;
+    return PBF_OK;
+}
+
+int pbf_mlp_get_params(pbf_mlp* m, float* params) {
+    if (!m || !params) return set_error(PBF_ERR_ARG, "NULL argument");
+    HIP_TRY(hipSetDevice(m->device));
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    HIP_TRY(hipMemcpy(params, m->p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
+    return PBF_OK;
+}
+
+int pbf_mlp_set_params(pbf_mlp* m, const float* params) {
+    if (!m || !params) return set_error(PBF_ERR_ARG, "NULL argument");
+    HIP_TRY(hipSetDevice(m->device));
+    HIP_TRY(hipMemcpyAsync(m->p, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, m->stream));
+    launch_mlp_pack(m->stream, m->p, m->M);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    return PBF_OK;
+}
+
+int pbf_mlp_set_normalization(pbf_mlp* m, const float* feat_mean, const float* feat_inv_std, float yscale,
+                              float yshift) {
+    if (!m || !feat_mean || !feat_inv_std) return set_error(PBF_ERR_ARG, "NULL argument");
+    HIP_TRY(hipSetDevice(m->device));
+    float ys[2] = {yscale, yshift};
+    HIP_TRY(hipMemcpy(m->M->fmean, feat_mean, sizeof(float) * kIn, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(m->M->finv, feat_inv_std, sizeof(float) * kIn, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(&m->M->yscale, ys, sizeof(ys), hipMemcpyHostToDevice));
+    return PBF_OK;
+}
+
+int pbf_mlp_forward(pbf_mlp* m, const float* x, int32_t n, float* y) {
+    if (!m || (n > 0 && (!x || !y)) || n < 0) return set_error(PBF_ERR_ARG, "bad argument");
+    if (n == 0) return PBF_OK;
+    HIP_TRY(hipSetDevice(m->device));
+    float *dx = nullptr, *dy = nullptr;
+    HIP_TRY(hipMalloc(&dx, sizeof(float) * (size_t)n * kIn));
+    hipError_t e = hipMalloc(&dy, sizeof(float) * (size_t)n);
+    if (e == hipSuccess) e = hipMemcpyAsync(dx, x, sizeof(float) * (size_t)n * kIn, hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess) {
+        launch_mlp_infer(m->stream, m->M, dx, n, nullptr, dy, 1.0f);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(y, dy, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    (void)hipFree(dx);
+    if (dy) (void)hipFree(dy);
+    if (e != hipSuccess) return set_error(PBF_ERR_HIP, "mlp forward: %s", hipGetErrorString(e));
+    return PBF_OK;
+}
+
+// Gradient of the mean squared error over (x, t) with respect to the packed parameters.
+int pbf_mlp_loss_grad(pbf_mlp* m, const float* x, const float* t, int32_t n, float* grad, float* loss) {
+    if (!m || n <= 0 || !x || !t) return set_error(PBF_ERR_ARG, "bad argument");
+    HIP_TRY(hipSetDevice(m->device));
+    int rc = mlp_ensure_idx(m, n);
+    if (rc) return rc;
+    float *dx = nullptr, *dt = nullptr;
+    HIP_TRY(hipMalloc(&dx, sizeof(float) * (size_t)n * kIn));
+    hipError_t e = hipMalloc(&dt, sizeof(float) * (size_t)n);
+    std::vector<uint32_t> id(n);
+    for (int i = 0; i < n; ++i) id[i] = (uint32_t)i;
+    if (e == hipSuccess) e = hipMemcpyAsync(dx, x, sizeof(float) * (size_t)n * kIn, hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dt, t, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(m->idx, id.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess) {
+        launch_mlp_grad(m->stream, m->M, dx, dt, m->idx, n, m->partial, m->g);
+        e = hipGetLastError();
+    }
+    std::vector<float> g(kMlpParams + 1);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(g.data(), m->g, sizeof(float) * (kMlpParams + 1), hipMemcpyDeviceToHost, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    (void)hipFree(dx);
+    if (dt) (void)hipFree(dt);
+    if (e != hipSuccess) return set_error(PBF_ERR_HIP, "mlp grad: %s", hipGetErrorString(e));
+    if (grad) std::memcpy(grad, g.data(), sizeof(float) * kMlpParams);
+    if (loss) *loss = g[kMlpParams] / (float)n;
+    return PBF_OK;
+}
+
+// One Adam step with the gradient of the last pbf_mlp_loss_grad (grad == NULL) or the given one.
+int pbf_mlp_adam_step(pbf_mlp* m, const float* grad, float lr, float beta1, float beta2, float eps) {
+    if (!m) return set_error(PBF_ERR_ARG, "NULL argument");
+    HIP_TRY(hipSetDevice(m->device));
+    if (grad) HIP_TRY(hipMemcpyAsync(m->g, grad, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, m->stream));
+    m->adam_t++;
+    float c1 = (float)(1.0 - std::pow((double)beta1, m->adam_t));
+    float c2 = (float)(1.0 - std::pow((double)beta2, m->adam_t));
+    launch_adam(m->stream, m->p, m->m, m->v, m->g, lr, beta1, beta2, eps, c1, c2);
+    launch_mlp_pack(m->stream, m->p, m->M);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    return PBF_OK;
+}
+
+// Trains on a dataset already in device memory (X [nsamples][16] raw features, T [nsamples]
+// normalised targets): `steps` Adam steps on batches drawn with SplitMix64(seed).  losses[i]
+// (host, may be NULL) receives the batch loss of step i.  Learning rate decays by cosine to
+// lr_final.
+int pbf_mlp_train(pbf_mlp* m, const float* dev_x, const float* dev_t, int64_t nsamples, int32_t batch, int32_t steps,
+                  float lr, float lr_final, uint64_t seed, float* losses) {
+    if (!m || !dev_x || !dev_t || nsamples <= 0 || batch <= 0 || steps < 0)
+        return set_error(PBF_ERR_ARG, "bad argument");
+    HIP_TRY(hipSetDevice(m->device));
+    int rc = mlp_ensure_idx(m, batch);
+    if (rc) return rc;
+    float* dloss = nullptr;
+    HIP_TRY(hipMalloc(&dloss, sizeof(float) * (size_t)std::max(steps, 1)));
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    hipError_t e = hipSuccess;
+    for (int it = 0; it < steps && e == hipSuccess; ++it) {
+        launch_batch_indices(m->stream, seed, (uint64_t)it, batch, (uint64_t)nsamples, m->idx);
+        launch_mlp_grad(m->stream, m->M, dev_x, dev_t, m->idx, batch, m->partial, m->g);
+        e = hipMemcpyAsync(dloss + it, m->g + kMlpParams, sizeof(float), hipMemcpyDeviceToDevice, m->stream);
+        m->adam_t++;
+        float c1 = (float)(1.0 - std::pow((double)b1, m->adam_t));
+        float c2 = (float)(1.0 - std::pow((double)b2, m->adam_t));
+        double frac = steps > 1 ? (double)it / (double)(steps - 1) : 1.0;
+        float lr_t = (float)(lr_final + 0.5 * (lr - lr_final) * (1.0 + std::cos(3.14159265358979 * frac)));
+        launch_adam(m->stream, m->p, m->m, m->v, m->g, lr_t, b1, b2, eps, c1, c2);
+        launch_mlp_pack(m->stream, m->p, m->M);
+        if (e == hipSuccess) e = hipGetLastError();
+    }
+    if (e == hipSuccess && losses && steps > 0) {
+        std::vector<float> l(steps);
+        e = hipMemcpyAsync(l.data(), dloss, sizeof(float) * steps, hipMemcpyDeviceToHost, m->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+        for (int i = 0; i < steps; ++i) losses[i] = l[i] / (float)batch;
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    (void)hipFree(dloss);
+    if (e != hipSuccess) return set_error(PBF_ERR_HIP, "mlp train: %s", hipGetErrorString(e));
+    return PBF_OK;
+}
+
+int pbf_attach_mlp(pbf_solver* s, pbf_mlp* m) {
+    if (!s) return set_error(PBF_ERR_ARG, "solver is NULL");
+    if (m && m->device != s->device) return set_error(PBF_ERR_ARG, "network and solver are on different devices");
+    if (m) HIP_TRY(hipStreamSynchronize(m->stream));
+    s->mlp = m ? m->M : nullptr;
+    return PBF_OK;
+}
+
+// Device memory helpers for callers without an allocator of their own (datasets, exports).
+int pbf_device_alloc(uint64_t bytes, void** out) {
+    if (!out) return set_error(PBF_ERR_ARG, "out is NULL");
+    HIP_TRY(hipMalloc(out, std::max<uint64_t>(bytes, 256)));
+    return PBF_OK;
+}
+
+int pbf_device_free(void* p) {
+    if (p) HIP_TRY(hipFree(p));
+    return PBF_OK;
+}
+
+int pbf_memcpy_to_host(void* dst, const void* src, uint64_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return PBF_OK;
+}
+
+int pbf_memcpy_to_device(void* dst, const void* src, uint64_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return PBF_OK;
+}
+
+// One plain step with `iterations` Jacobi iterations that also records, for every particle
+// with id % stride == phase, its raw features (at the predicted positions) and the multiplier
+// the step accumulated over its iterations divided by h^2, at row base + id / stride of
+// (dev_x [cap][16], dev_t [cap]).
+int pbf_record_step(pbf_solver* s, int32_t iterations, float* dev_x, float* dev_t, int64_t base, int64_t cap,
+                    int32_t stride, int32_t phase) {
+    if (!s || !dev_x || !dev_t || iterations < 1 || stride < 1 || phase < 0 || phase >= stride || base < 0)
+        return set_error(PBF_ERR_ARG, "bad argument");
+    HIP_TRY(hipSetDevice(s->device));
+    float lam_min = 0.0f;
+    int32_t used = 0;
+    int rc = pbf_step_adaptive(s, 0, lam_min, iterations, iterations, &used, nullptr);
+    if (rc) return rc;
+    const float inv_h2 = 1.0f / (s->P.h * s->P.h);
+    launch_record(s->stream, s->n, s->B.ids[s->ic], s->feat, s->lam_sum, inv_h2, stride, phase, (size_t)base,
+                  (size_t)cap, dev_x, dev_t);
+    HIP_TRY(hipGetLastError());
+    return PBF_OK;
+}
+
+// One step whose constraint loop stops as soon as the compression residual (mean of
+// max(C_i, 0) over the frozen lists at the current positions) is <= tol after at least
+// min_iters iterations, or after max_iters.
+//
+// With use_mlp (a network attached) iteration 1 starts from the network's multiplier.  The guess
+// is kept when its residual is <= tol, or below the residual of the predicted state (then plain
+// iterations continue from it, the fallback that judges its error).  Otherwise it is rejected:
+// lambda is recomputed and the step runs plain from the predicted state, exactly as without the
+// network.  iters_used / resid (may be NULL) report the iterations run (the guess counts as one)
+// and the last residual measured.
+int pbf_step_adaptive(pbf_solver* s, int32_t use_mlp, float tol, int32_t min_iters, int32_t max_iters,
+                      int32_t* iters_used, float* resid) {
+    if (!s || max_iters < 1 || min_iters < 0) return set_error(PBF_ERR_ARG, "bad argument");
+    if (use_mlp && !s->mlp) return set_error(PBF_ERR_STATE, "no network attached");
+    HIP_TRY(hipSetDevice(s->device));
+    const StepConsts K = make_consts(s);
+    StepBuffers& B = s->B;
+    hipStream_t st = s->stream;
+    const bool dbg = (s->P.flags & PBF_FLAG_DEBUG) != 0;
+    const bool stats = dbg || (s->P.flags & PBF_FLAG_DENSITY_STATS);
+    launch_predict_hash(st, K, B.pos[s->pc], B.vel[s->vc], B);
+    launch_scan(st, K, B);
+    launch_sort_reorder(st, K, B, B.pos[s->pc], B.ids[s->ic], B.pos[s->pc ^ 1], B.vel[s->vc ^ 1], B.ids[s->ic ^ 1],
+                        dbg ? s->dbg_gid : nullptr);
+    int a = s->pc ^ 1;
+    const int v = s->vc ^ 1;
+    LearnArgs LF{};
+    LF.feat = s->feat;
+    LF.vel = B.vel[v];
+    LF.resid = use_mlp ? s->resid0 : nullptr;
+    launch_neighbors_lambda(st, K, B.pos[a], B, dbg ? s->dbg_lam : nullptr, LF);
+    LearnArgs LA{};
+    LA.lam_sum = s->lam_sum;
+    LearnArgs LR{};
+    LR.resid = s->resid;
+    int it = 0;
+    float r = -1.0f;
+    bool done = false;
+    if (use_mlp) {
+        launch_mlp_infer(st, s->mlp, s->feat, s->n, B.pos[a], nullptr, s->P.h * s->P.h);
+        launch_delta_apply(st, K, B.pos[a], B.pos[a ^ 1], B.vel[v], B, 0, 1, LA);
+        launch_lambda(st, K, B.pos[a ^ 1], B, LR);
+        HIP_TRY(hipGetLastError());
+        float r0 = 0.0f, rg = 0.0f;
+        int rc = read_residual(s, s->resid0, &r0);
+        if (!rc) rc = read_residual(s, s->resid, &rg);
+        if (rc) return rc;
+        s->last_guess_resid = rg;
+        s->last_pred_resid = r0;
+        if ((rg <= tol && min_iters <= 1) || rg < r0) {
+            a ^= 1;
+            it = 1;
+            r = rg;
+            s->guess_accepted = 1;
+            done = (rg <= tol && min_iters <= 1) || it >= max_iters;
+        } else {
+            // rejected: restore lambda of iteration 1 at the predicted positions and run plain
+            LearnArgs LN{};
+            launch_neighbors_lambda(st, K, B.pos[a], B, nullptr, LN);
+            s->guess_accepted = 0;
+        }
+    }
+    while (!done) {
+        launch_delta_apply(st, K, B.pos[a], B.pos[a ^ 1], B.vel[v], B, 0, it == 0, LA);
+        a ^= 1;
+        ++it;
+        if (it >= max_iters) break;
+        launch_lambda(st, K, B.pos[a], B, LR);
+        HIP_TRY(hipGetLastError());
+        int rc = read_residual(s, s->resid, &r);
+        if (rc) return rc;
+        if (it >= min_iters && r <= tol) break;
+    }
+    launch_velocity(st, K, B.pos[a], B.vel[v], B);
+    launch_curl(st, K, B.pos[a], B.vel[v], B, dbg ? s->dbg_err : nullptr, stats ? s->partial : nullptr);
+    launch_vort_apply(st, K, B.pos[a], B.vel[v], B.vel[v ^ 1], B, s->lam_sum);
+    HIP_TRY(hipGetLastError());
+    s->lam_last = a ^ 1;
+    s->pc = a;
+    s->vc = v ^ 1;
+    s->ic ^= 1;
+    s->stats_valid = stats;
+    s->steps_done++;
+    if (iters_used) *iters_used = it;
+    if (resid) *resid = r;
+    return PBF_OK;
+}
+
+// Residuals of the last learned step: out[0] = predicted state, out[1] = after the guess,
+// out[2] = 1 if the guess was kept.
+int pbf_last_guess_info(pbf_solver* s, float out[3]) {
+    if (!s || !out) return set_error(PBF_ERR_ARG, "NULL argument");
+    out[0] = s->last_pred_resid;
+    out[1] = s->last_guess_resid;
+    out[2] = (float)s->guess_accepted;
+    return PBF_OK;
 }
 
 }  // extern "C"

@@ -27,6 +27,22 @@ struct StepBuffers {
     uint8_t* ncnt;
 };
 
+// Warm-start features per particle (raw fp32, written by the neighbour pass when requested):
+//  0 C_i = rho/rho0 - 1 at the predicted position   1 neighbour count
+//  2 sum_j |grad_j C_i|^2                             3 |grad_i C_i|
+//  4-6 v * dt / h (predicted velocity)                7 previous step's multiplier / h^2
+//  8 distance to the nearest wall / h (<= 4)          9-14 distance to each wall (x-,x+,y-,y+,z-,z+) / h
+//  15 lambda of iteration 1 / h^2
+constexpr int kNumFeatures = 16;
+
+// Optional outputs of the constraint kernels (all NULL on the plain path).
+struct LearnArgs {
+    float* feat;         // [n][kNumFeatures] raw features (neighbour pass)
+    const float4* vel;   // predicted velocity, multiplier of the previous step in .w (features)
+    float* lam_sum;      // accumulated multiplier of this step (delta-p passes)
+    float* resid;        // block sums of max(C, 0) (lambda passes)
+};
+
 int scan_tiles(int m);
 int err_partial_blocks(int n);
 
@@ -39,14 +55,16 @@ void launch_scan(hipStream_t st, const StepConsts& K, const StepBuffers& B);
 void launch_sort_reorder(hipStream_t st, const StepConsts& K, const StepBuffers& B, const float4* pos_in,
                          const uint32_t* ids_in, float4* pos_out, float4* vel_out, uint32_t* ids_out,
                          int32_t* dbg_gid);
-void launch_neighbors_lambda(hipStream_t st, const StepConsts& K, float4* pos, const StepBuffers& B, float* dbg_lam);
-void launch_lambda(hipStream_t st, const StepConsts& K, float4* pos, const StepBuffers& B);
+void launch_neighbors_lambda(hipStream_t st, const StepConsts& K, float4* pos, const StepBuffers& B, float* dbg_lam,
+                             const LearnArgs& L);
+void launch_lambda(hipStream_t st, const StepConsts& K, float4* pos, const StepBuffers& B, const LearnArgs& L);
 void launch_delta_apply(hipStream_t st, const StepConsts& K, const float4* pos, float4* pos_next, float4* vel,
-                        const StepBuffers& B, int last);
+                        const StepBuffers& B, int last, int first, const LearnArgs& L);
 void launch_curl(hipStream_t st, const StepConsts& K, const float4* pos, const float4* vel, const StepBuffers& B,
                  float* err, float* partial);
 void launch_vort_apply(hipStream_t st, const StepConsts& K, const float4* pos, const float4* vel, float4* vel_out,
-                       const StepBuffers& B);
+                       const StepBuffers& B, const float* lam_sum);
+void launch_velocity(hipStream_t st, const StepConsts& K, const float4* pos, float4* vel, const StepBuffers& B);
 void launch_export(hipStream_t st, int n, const float4* pos, const float4* vel, const uint32_t* ids, float* pos_xyz,
                    float* vel_xyz);
 void launch_import(hipStream_t st, int n, const float* pos_xyz, const float* vel_xyz, float4* pos, float4* vel,

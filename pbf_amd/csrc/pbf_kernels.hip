@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kBlock) void k_predict_hash(StepConsts K, const flo
     int c = cell_of(x.x, x.y, x.z, K);
     uint32_t l = atomicAdd(&count[c], 1u);
     xp[s] = make_float4(x.x, x.y, x.z, 0.0f);
-    vp[s] = make_float4(v.x, v.y, v.z, 0.0f);
+    vp[s] = v;  // .w carries the previous step's accumulated multiplier (warm start)
     cell[s] = (uint32_t)c;
     local[s] = l;
 }
@@ -218,15 +218,33 @@ struct LambdaAcc {
     }
 };
 
+// Block sum of max(C, 0) (the compression residual) in a fixed tree order.
+__device__ void block_residual(float c, float* __restrict__ partial) {
+    __shared__ float sr[kBlock];
+    int t = threadIdx.x;
+    sr[t] = c;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (t < w) sr[t] = sr[t] + sr[t + w];
+        __syncthreads();
+    }
+    if (t == 0) partial[blockIdx.x] = sr[0];
+}
+
 // buildNeighborhood (cu:124-154) fused with the first calculateLambda: the 27 cells in z, y, x
 // order (the three x-neighbours of a row are one contiguous slot run), self excluded,
-// r.r <= h*h, at most cap hits.  Lambda of iteration 1 goes to pos.w.
+// r.r <= h*h, at most cap hits.  Lambda of iteration 1 goes to pos.w.  With L.feat the raw
+// warm-start features of the particle are written as well (kNumFeatures floats, see
+// pbf_kernels.h); with L.resid the block sums of max(C, 0) at the predicted positions.
 __global__ __launch_bounds__(kBlock) void k_neighbors_lambda(StepConsts K, float4* __restrict__ pos,
                                                              const uint32_t* __restrict__ cstart,
                                                              uint32_t* __restrict__ nbr, uint8_t* __restrict__ ncnt,
-                                                             float* __restrict__ dbg_lam) {
+                                                             float* __restrict__ dbg_lam, LearnArgs L) {
     int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= K.n) return;
+    if (s >= K.n) {
+        if (L.resid) block_residual(0.0f, L.resid);
+        return;
+    }
     const float4 xi = pos[s];
     int d0 = dim_of(xi.x, K.gmin[0], K.invH, K.dims[0]);
     int d1 = dim_of(xi.y, K.gmin[1], K.invH, K.dims[1]);
@@ -261,26 +279,61 @@ __global__ __launch_bounds__(kBlock) void k_neighbors_lambda(StepConsts K, float
     float lam = A.lambda(K);
     reinterpret_cast<float*>(pos + s)[3] = lam;
     if (dbg_lam) dbg_lam[s] = lam;
+    if (L.feat) {
+        const float4 v = L.vel[s];
+        const float ih = 1.0f / K.h, ih2 = ih * ih;
+        float f[kNumFeatures];
+        f[0] = A.rho / K.restRho - 1.0f;
+        f[1] = (float)cnt;
+        f[2] = A.gms / (K.restRho * K.restRho);
+        f[3] = sqrtf(A.gx * A.gx + A.gy * A.gy + A.gz * A.gz) / K.restRho;
+        f[4] = v.x * K.dt * ih;
+        f[5] = v.y * K.dt * ih;
+        f[6] = v.z * K.dt * ih;
+        f[7] = v.w * ih2;
+        float dmin = 4.0f;
+        const float p3[3] = {xi.x, xi.y, xi.z};
+        for (int a = 0; a < 3; ++a) {
+            float lo = fminf(fmaxf((p3[a] - K.bmin[a]) * ih, 0.0f), 4.0f);
+            float hi = fminf(fmaxf((K.bmax[a] - p3[a]) * ih, 0.0f), 4.0f);
+            f[9 + 2 * a] = lo;
+            f[10 + 2 * a] = hi;
+            dmin = fminf(dmin, fminf(lo, hi));
+        }
+        f[8] = dmin;
+        f[15] = lam * ih2;
+        float4* o = reinterpret_cast<float4*>(L.feat + (size_t)s * kNumFeatures);
+        o[0] = make_float4(f[0], f[1], f[2], f[3]);
+        o[1] = make_float4(f[4], f[5], f[6], f[7]);
+        o[2] = make_float4(f[8], f[9], f[10], f[11]);
+        o[3] = make_float4(f[12], f[13], f[14], f[15]);
+    }
+    if (L.resid) block_residual(fmaxf(A.rho / K.restRho - 1.0f, 0.0f), L.resid);
 }
 
-// calculateLambda (cu:156-180) over the frozen list, iterations 2..I.
+// calculateLambda (cu:156-180) over the frozen list, iterations 2..I.  With L.resid, the block
+// sums of max(C_i, 0) at the current positions are written as well.
 __global__ __launch_bounds__(kBlock) void k_lambda(StepConsts K, float4* __restrict__ pos,
                                                    const uint32_t* __restrict__ nbr,
-                                                   const uint8_t* __restrict__ ncnt) {
+                                                   const uint8_t* __restrict__ ncnt, LearnArgs L) {
     int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= K.n) return;
-    const float4 xi = pos[s];
-    int cnt = ncnt[s];
-    LambdaAcc A;
-    A.init();
-    const uint32_t* L = nbr + s;
-    for (int j = 0; j < cnt; ++j) {
-        float4 xj = pos[L[(size_t)j * K.nstride]];
-        float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-        float r2 = rx * rx + ry * ry + rz * rz;
-        A.add(rx, ry, rz, r2, K);
+    float comp = 0.0f;
+    if (s < K.n) {
+        const float4 xi = pos[s];
+        int cnt = ncnt[s];
+        LambdaAcc A;
+        A.init();
+        const uint32_t* Lst = nbr + s;
+        for (int j = 0; j < cnt; ++j) {
+            float4 xj = pos[Lst[(size_t)j * K.nstride]];
+            float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+            float r2 = rx * rx + ry * ry + rz * rz;
+            A.add(rx, ry, rz, r2, K);
+        }
+        reinterpret_cast<float*>(pos + s)[3] = A.lambda(K);
+        comp = fmaxf(A.rho / K.restRho - 1.0f, 0.0f);
     }
-    reinterpret_cast<float*>(pos + s)[3] = A.lambda(K);
+    if (L.resid) block_residual(comp, L.resid);
 }
 
 // calculateOffset (cu:182-199) + updatePosition (cu:15-22): delta-p with s_corr, then
@@ -291,11 +344,12 @@ __global__ __launch_bounds__(kBlock) void k_delta_apply(StepConsts K, const floa
                                                         const uint32_t* __restrict__ nbr,
                                                         const uint8_t* __restrict__ ncnt,
                                                         const float4* __restrict__ xold, float4* __restrict__ vel,
-                                                        int last) {
+                                                        int last, int first, LearnArgs LA) {
     int s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= K.n) return;
     const float4 xi = pos[s];
     int cnt = ncnt[s];
+    if (LA.lam_sum) LA.lam_sum[s] = (first ? 0.0f : LA.lam_sum[s]) + xi.w;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     const uint32_t* L = nbr + s;
     for (int j = 0; j < cnt; ++j) {
@@ -385,7 +439,8 @@ __global__ __launch_bounds__(kBlock) void k_vort_apply(StepConsts K, const float
                                                        const float4* __restrict__ vel,
                                                        const float4* __restrict__ omega,
                                                        const uint32_t* __restrict__ nbr,
-                                                       const uint8_t* __restrict__ ncnt, float4* __restrict__ vel_out) {
+                                                       const uint8_t* __restrict__ ncnt, float4* __restrict__ vel_out,
+                                                       const float* __restrict__ lam_sum) {
     int s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= K.n) return;
     const float4 xi = pos[s];
@@ -422,7 +477,17 @@ __global__ __launch_bounds__(kBlock) void k_vort_apply(StepConsts K, const float
     float cz = ex * wi.y - ey * wi.x;
     float fx = K.vort_eps * cx, fy = K.vort_eps * cy, fz = K.vort_eps * cz;
     float vx = K.visc_c * px, vy = K.visc_c * py, vz = K.visc_c * pz;
-    vel_out[s] = make_float4(vi.x + (K.dt * fx + vx), vi.y + (K.dt * fy + vy), vi.z + (K.dt * fz + vz), 0.0f);
+    vel_out[s] = make_float4(vi.x + (K.dt * fx + vx), vi.y + (K.dt * fy + vy), vi.z + (K.dt * fz + vz),
+                             lam_sum ? lam_sum[s] : 0.0f);
+}
+
+// updatePosition's velocity (cu:21) for a constraint loop that stopped early.
+__global__ __launch_bounds__(kBlock) void k_velocity(StepConsts K, const float4* __restrict__ pos,
+                                                     const float4* __restrict__ xold, float4* __restrict__ vel) {
+    int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= K.n) return;
+    float4 x = pos[s], o = xold[s];
+    vel[s] = make_float4((x.x - o.x) / K.dt, (x.y - o.y) / K.dt, (x.z - o.z) / K.dt, 0.0f);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -493,21 +558,22 @@ void launch_sort_reorder(hipStream_t st, const StepConsts& K, const StepBuffers&
                        B.xp, B.vp, pos_in, pos_out, vel_out, B.xold, ids_out, dbg_gid);
 }
 
-void launch_neighbors_lambda(hipStream_t st, const StepConsts& K, float4* pos, const StepBuffers& B, float* dbg_lam) {
+void launch_neighbors_lambda(hipStream_t st, const StepConsts& K, float4* pos, const StepBuffers& B, float* dbg_lam,
+                             const LearnArgs& L) {
     if (K.n > 0)
         hipLaunchKernelGGL(k_neighbors_lambda, dim3(nblk(K.n)), dim3(kBlock), 0, st, K, pos, B.cstart, B.nbr, B.ncnt,
-                           dbg_lam);
+                           dbg_lam, L);
 }
 
-void launch_lambda(hipStream_t st, const StepConsts& K, float4* pos, const StepBuffers& B) {
-    if (K.n > 0) hipLaunchKernelGGL(k_lambda, dim3(nblk(K.n)), dim3(kBlock), 0, st, K, pos, B.nbr, B.ncnt);
+void launch_lambda(hipStream_t st, const StepConsts& K, float4* pos, const StepBuffers& B, const LearnArgs& L) {
+    if (K.n > 0) hipLaunchKernelGGL(k_lambda, dim3(nblk(K.n)), dim3(kBlock), 0, st, K, pos, B.nbr, B.ncnt, L);
 }
 
 void launch_delta_apply(hipStream_t st, const StepConsts& K, const float4* pos, float4* pos_next, float4* vel,
-                        const StepBuffers& B, int last) {
+                        const StepBuffers& B, int last, int first, const LearnArgs& L) {
     if (K.n > 0)
         hipLaunchKernelGGL(k_delta_apply, dim3(nblk(K.n)), dim3(kBlock), 0, st, K, pos, pos_next, B.nbr, B.ncnt,
-                           B.xold, vel, last);
+                           B.xold, vel, last, first, L);
 }
 
 void launch_curl(hipStream_t st, const StepConsts& K, const float4* pos, const float4* vel, const StepBuffers& B,
@@ -518,10 +584,14 @@ void launch_curl(hipStream_t st, const StepConsts& K, const float4* pos, const f
 }
 
 void launch_vort_apply(hipStream_t st, const StepConsts& K, const float4* pos, const float4* vel, float4* vel_out,
-                       const StepBuffers& B) {
+                       const StepBuffers& B, const float* lam_sum) {
     if (K.n > 0)
         hipLaunchKernelGGL(k_vort_apply, dim3(nblk(K.n)), dim3(kBlock), 0, st, K, pos, vel, B.omega, B.nbr, B.ncnt,
-                           vel_out);
+                           vel_out, lam_sum);
+}
+
+void launch_velocity(hipStream_t st, const StepConsts& K, const float4* pos, float4* vel, const StepBuffers& B) {
+    if (K.n > 0) hipLaunchKernelGGL(k_velocity, dim3(nblk(K.n)), dim3(kBlock), 0, st, K, pos, B.xold, vel);
 }
 
 int err_partial_blocks(int n) { return (int)nblk(n); }

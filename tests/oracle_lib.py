@@ -60,6 +60,11 @@ def orc() -> C.CDLL:
             "orc_get_neighbors": (None, [_VP, _I, _I]),
             "orc_get_fields": (None, [_VP, _F, _F, _F, _F]),
             "orc_get_dp": (None, [_VP, _F]),
+            "mlp_param_count": (C.c_int, []),
+            "mlp_bf16": (C.c_float, [C.c_float]),
+            "mlp_forward": (None, [_F, _F, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+            "mlp_loss_grad": (C.c_double, [_F, _F, _F, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+            "mlp_adam": (None, [_F, _F, _F, _F, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -162,6 +167,48 @@ class Oracle:
         cu = np.empty((self.n, 3), np.float32)
         orc().orc_get_fields(self._h, fp(lf), fp(ll), fp(er), fp(cu))
         return lf, ll, er, cu
+
+
+def dp_(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def mlp_forward(params, x, bf16=True):
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, 16)
+    y = np.empty(x.shape[0], np.float64)
+    orc().mlp_forward(fp(params), fp(x), x.shape[0], int(bf16), dp_(y))
+    return y
+
+
+def mlp_loss_grad(params, x, t, bf16=True):
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, 16)
+    t = np.ascontiguousarray(t, np.float32)
+    g = np.empty(orc().mlp_param_count(), np.float64)
+    loss = orc().mlp_loss_grad(fp(params), fp(x), fp(t), x.shape[0], int(bf16), dp_(g))
+    return loss, g
+
+
+def mlp_adam(params, m, v, grad, t, lr, b1=0.9, b2=0.999, eps=1e-8):
+    params = np.ascontiguousarray(params, np.float32).copy()
+    m = np.ascontiguousarray(m, np.float32).copy()
+    v = np.ascontiguousarray(v, np.float32).copy()
+    grad = np.ascontiguousarray(grad, np.float32)
+    orc().mlp_adam(fp(params), fp(m), fp(v), fp(grad), params.size, int(t), lr, b1, b2, eps)
+    return params, m, v
+
+
+MLP_SLICES = {
+    "W1": slice(0, 1024),
+    "b1": slice(1024, 1088),
+    "W2": slice(1088, 5184),
+    "b2": slice(5184, 5248),
+    "W3": slice(5248, 9344),
+    "b3": slice(9344, 9408),
+    "W4": slice(9408, 9472),
+    "b4": slice(9472, 9473),
+}
 
 
 def density_stats(err: np.ndarray):

@@ -1,0 +1,142 @@
+"""GPU tests of the learned warm start: the BF16 MFMA network against the FP64 oracle (forward,
+gradients, Adam), device training, dataset recording, the adaptive loop and its fallback."""
+import numpy as np
+import pytest
+
+from oracle_lib import MLP_SLICES, Oracle, mlp_adam, mlp_forward, mlp_loss_grad
+from pbf_amd import FLAG_DEBUG, DeviceDataset, Mlp, Solver, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def random_params(seed, scale=0.25):
+    rng = np.random.default_rng(seed)
+    p = (rng.standard_normal(9473) * scale).astype(np.float32)
+    return p
+
+
+def identity_net(seed=3):
+    m = Mlp(seed=seed)
+    p = random_params(seed)
+    m.set_params(p)
+    m.set_normalization(np.zeros(16), np.ones(16), 1.0, 0.0)
+    return m, p
+
+
+def test_forward_matches_oracle():
+    print("[case] mlp forward", flush=True)
+    m, p = identity_net(3)
+    rng = np.random.default_rng(4)
+    x = np.clip(rng.standard_normal((300, 16)), -7.5, 7.5).astype(np.float32)
+    y = m.forward(x)
+    ref = mlp_forward(p, x, bf16=True)
+    scale = np.abs(ref).max()
+    # bf16 activations: an fp32 pre-activation that differs in its last bits from the FP64 one
+    # can round to the neighbouring bf16 value; the bound allows a few such flips.
+    assert np.abs(y - ref).max() <= 2e-2 * scale
+    assert np.median(np.abs(y - ref)) <= 2e-3 * scale
+
+
+def test_gradients_match_oracle():
+    print("[case] mlp gradients", flush=True)
+    m, p = identity_net(5)
+    rng = np.random.default_rng(6)
+    x = np.clip(rng.standard_normal((1000, 16)), -7.5, 7.5).astype(np.float32)
+    t = rng.standard_normal(1000).astype(np.float32)
+    loss, g = m.loss_grad(x, t)
+    loss_ref, g_ref = mlp_loss_grad(p, x, t, bf16=True)
+    assert loss == pytest.approx(loss_ref, rel=1e-2)
+    for name, sl in MLP_SLICES.items():
+        a, b = g[sl], g_ref[sl]
+        scale = np.abs(b).max()
+        # BF16 operands in the backward GEMMs: agree to a few bf16 ulps of the tensor's scale
+        assert np.abs(a - b).max() <= 5e-2 * scale, name
+        assert np.corrcoef(a, b)[0, 1] > 0.999 if a.size > 1 else True, name
+
+
+def test_adam_matches_oracle():
+    print("[case] adam", flush=True)
+    m, p = identity_net(7)
+    rng = np.random.default_rng(8)
+    g = rng.standard_normal(9473).astype(np.float32)
+    m.adam_step(g, lr=1e-3)
+    ref, _, _ = mlp_adam(p, np.zeros_like(p), np.zeros_like(p), g, 1, 1e-3)
+    np.testing.assert_allclose(m.get_params(), ref, rtol=1e-5, atol=1e-7)
+
+
+def test_training_fits_a_smooth_function():
+    print("[case] mlp training", flush=True)
+    rng = np.random.default_rng(9)
+    n = 1 << 15
+    x = rng.uniform(-2, 2, size=(n, 16)).astype(np.float32)
+    t = (np.sin(x[:, 0]) + 0.5 * x[:, 1] * x[:, 2] - 0.3 * x[:, 3] ** 2 + 0.2 * x[:, 4:].sum(1)).astype(np.float32)
+    ds = DeviceDataset(n)
+    ds.upload(x, t)
+    m = Mlp(seed=1)
+    m.set_normalization(np.zeros(16), np.ones(16), 1.0, 0.0)
+    losses = m.train(ds, n, batch=1024, steps=600, lr=3e-3, lr_final=1e-4, seed=2)
+    assert np.isfinite(losses).all()
+    assert losses[-50:].mean() < 0.1 * losses[:10].mean()
+    y = m.forward(x[:2000])
+    assert np.mean((y - t[:2000]) ** 2) < 0.1 * np.var(t)
+
+
+def pair_steps(sc, **over):
+    return Solver.from_scene(sc, **over), Solver.from_scene(sc, **over)
+
+
+def test_adaptive_fixed_iterations_equal_plain():
+    """min_iters = max_iters = I runs exactly the plain step (bit for bit)."""
+    print("[case] adaptive == plain", flush=True)
+    sc = scenes.config_cpu_32k()
+    a, b = pair_steps(sc)
+    for _ in range(3):
+        a.step()
+        it, r = b.step_adaptive(False, 0.0, 4, 4)
+        assert it == 4
+    pa, va = a.get_state()
+    pb, vb = b.get_state()
+    assert np.array_equal(pa.view(np.uint32), pb.view(np.uint32))
+    assert np.array_equal(va.view(np.uint32), vb.view(np.uint32))
+
+
+def test_garbage_network_is_rejected_and_reproduces_plain():
+    print("[case] garbage network fallback", flush=True)
+    sc = scenes.config_cpu_32k()
+    a, b = pair_steps(sc)
+    m = Mlp(seed=11)
+    m.set_params(random_params(11, scale=3.0))
+    m.set_normalization(np.zeros(16), np.ones(16), 1e3, 0.0)
+    b.attach_mlp(m)
+    for _ in range(3):
+        ia, ra = a.step_adaptive(False, 0.02, 1, 12)
+        ib, rb = b.step_adaptive(True, 0.02, 1, 12)
+        pred, guess, kept = b.last_guess_info()
+        assert not kept and guess > pred
+        assert ia == ib and ra == rb
+    pa, va = a.get_state()
+    pb, vb = b.get_state()
+    assert np.array_equal(pa.view(np.uint32), pb.view(np.uint32))
+    assert np.array_equal(va.view(np.uint32), vb.view(np.uint32))
+
+
+def test_record_step_features_and_targets():
+    """Features of the neighbour pass and the one-iteration target against the oracle."""
+    print("[case] record step", flush=True)
+    sc = scenes.dam_break((16, 16, 16), (0, 0, 0, 1.6, 1.6, 1.6), seed=21)
+    s = Solver.from_scene(sc, flags=FLAG_DEBUG)
+    o = Oracle.from_scene(sc, iterations=1)
+    ds = DeviceDataset(sc.n)
+    s.record_step(1, ds, base=0, stride=1, phase=0)
+    o.step(1)
+    x, t = ds.host(sc.n)
+    nbr, cnt = o.neighbors()
+    lam1 = o.fields()[0]
+    h = np.float32(sc.params["h"])
+    ih = np.float32(1.0) / h
+    assert np.array_equal(x[:, 1], cnt.astype(np.float32))
+    assert np.array_equal(x[:, 15], lam1 * (ih * ih))
+    inv_h2 = np.float32(1.0) / (h * h)
+    assert np.array_equal(t, lam1 * inv_h2)
+    assert (x[:, 8] >= 0).all() and (x[:, 8] <= 4).all()
+    np.testing.assert_array_equal(x[:, 8], np.minimum(np.min(x[:, 9:15], axis=1), 4.0))
