@@ -77,6 +77,27 @@ typedef struct pbf_solver pbf_solver;
 int pbf_create(const pbf_params* para, const float grid_space[6], const float boundary[6], const float* pos_xyz,
                const float* vel_xyz, int32_t n, pbf_solver** out);
 
+/* A batch of nscenes independent scenes (1..255) advanced by one solver: every step is one
+ * launch per kernel over all scenes.  counts[k] particles of scene k follow one another in
+ * pos_xyz / vel_xyz (the batch's particle order); grid_space[6k..6k+5], boundary[6k..6k+5] and
+ * grid_dims[3k..3k+2] are scene k's.  para is shared; its grid_dims are ignored.  Each scene
+ * evolves exactly as a pbf_create solver of that scene alone would.  No counterpart in the
+ * reference, which runs one fixed 64^3 scene. */
+int pbf_create_batch(const pbf_params* para, int32_t nscenes, const int32_t* counts, const float* grid_space,
+                     const float* boundary, const int32_t* grid_dims, const float* pos_xyz, const float* vel_xyz,
+                     pbf_solver** out);
+
+/* Launch shape of the neighbour kernels: block 64/128/256/512, grid > 0 = persistent grid of
+ * that many workgroups (0 = one per block), xcd_map = contiguous particle chunks per XCD.
+ * Results do not depend on it. */
+int pbf_set_tuning(pbf_solver* s, int32_t block, int32_t grid, int32_t xcd_map);
+int pbf_get_tuning(pbf_solver* s, int32_t out[3]);
+
+/* Autotuner: times every launch shape over `steps` graph-replayed steps, keeps the fastest and
+ * restores the state (tuning does not advance the simulation).  best[3] = block, grid,
+ * xcd_map; best_ms = time per step.  Either may be NULL. */
+int pbf_autotune(pbf_solver* s, int32_t steps, int32_t* best, float* best_ms);
+
 /* Replaces cudaPBFSolve (kernels/cudaPBF.h:12, kernels/cudaPBF.cu:50-114): one solver step
  * with para's values (dt, iterations, ...).  para->nParticles and para->grid_dims must equal
  * the solver's. */

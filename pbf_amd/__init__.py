@@ -164,6 +164,10 @@ def lib() -> C.CDLL:
         "pbf_record_step": (C.c_int, [_VP, C.c_int32, _VP, _VP, C.c_int64, C.c_int64, C.c_int32, C.c_int32]),
         "pbf_step_adaptive": (C.c_int, [_VP, C.c_int32, C.c_float, C.c_int32, C.c_int32, _I, _F]),
         "pbf_last_guess_info": (C.c_int, [_VP, _F]),
+        "pbf_create_batch": (C.c_int, [C.POINTER(SolverParameter), C.c_int32, _I, _F, _F, _I, _F, _F, C.POINTER(_VP)]),
+        "pbf_set_tuning": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32]),
+        "pbf_get_tuning": (C.c_int, [_VP, _I]),
+        "pbf_autotune": (C.c_int, [_VP, C.c_int32, _I, _F]),
         "pbf_device_alloc": (C.c_int, [C.c_uint64, C.POINTER(_VP)]),
         "pbf_device_free": (C.c_int, [_VP]),
         "pbf_memcpy_to_host": (C.c_int, [_VP, _VP, C.c_uint64]),
@@ -214,6 +218,47 @@ class Solver:
         h = _VP()
         _check(lib().pbf_create(C.byref(self.para), _fp(gs), _fp(bd), _fp(pos), _fp(vel), n, C.byref(h)))
         self._h = h
+
+    @classmethod
+    def batch(cls, scene_list, flags: int = 0, **over) -> "Solver":
+        """One solver advancing every scene of scene_list in each launch (pbf_create_batch).
+
+        Particles are the scenes' particles one scene after another; parameters are the first
+        scene's (with ``over``)."""
+        p = dict(scene_list[0].params)
+        p.update(over)
+        counts = np.array([sc.n for sc in scene_list], np.int32)
+        n = int(counts.sum())
+        para = SolverParameter.from_dict(p, n, scene_list[0].grid_dims, flags)
+        gs = np.ascontiguousarray(np.concatenate([sc.box for sc in scene_list]), np.float32)
+        bd = np.ascontiguousarray(np.concatenate([sc.boundary for sc in scene_list]), np.float32)
+        dims = np.ascontiguousarray(np.concatenate([np.array(sc.grid_dims, np.int32) for sc in scene_list]), np.int32)
+        pos = _f32(np.concatenate([sc.pos for sc in scene_list]), (n, 3))
+        vel = _f32(np.concatenate([sc.vel for sc in scene_list]), (n, 3))
+        self = cls.__new__(cls)
+        self.para = para
+        self.n = n
+        self.ncells = int(sum(int(np.prod(sc.grid_dims)) for sc in scene_list))
+        self.counts = counts
+        h = _VP()
+        _check(lib().pbf_create_batch(C.byref(self.para), len(scene_list), _ip(counts), _fp(gs), _fp(bd), _ip(dims), _fp(pos), _fp(vel), C.byref(h)))
+        self._h = h
+        return self
+
+    def set_tuning(self, block: int = 256, grid: int = 0, xcd_map: bool = True):
+        _check(lib().pbf_set_tuning(self._h, int(block), int(grid), int(bool(xcd_map))))
+
+    def get_tuning(self):
+        out = np.zeros(3, np.int32)
+        _check(lib().pbf_get_tuning(self._h, _ip(out)))
+        return int(out[0]), int(out[1]), bool(out[2])
+
+    def autotune(self, steps: int = 10):
+        """Picks the fastest launch shape (state unchanged); returns ((block, grid, xcd_map), ms/step)."""
+        best = np.zeros(3, np.int32)
+        ms = np.zeros(1, np.float32)
+        _check(lib().pbf_autotune(self._h, int(steps), _ip(best), _fp(ms)))
+        return (int(best[0]), int(best[1]), bool(best[2])), float(ms[0])
 
     @classmethod
     def from_scene(cls, scene, flags: int = 0, **over) -> "Solver":

@@ -58,6 +58,10 @@ struct pbf_solver {
     int device = 0;
     hipStream_t stream = nullptr;
     StepBuffers B{};
+    int nbuckets = 0;            // cell counters: ncells + 1, or the sum over a batch
+    Geometry geo{nullptr, nullptr};
+    int nscenes = 1;
+    Tuning tune{256, 0, 1};
     int pc = 0, vc = 0, ic = 0;  // which of pos/vel/ids[2] hold the state
     int lam_last = -1;           // pos buffer whose .w holds the last iteration's lambda
     int steps_done = 0;
@@ -126,6 +130,7 @@ StepConsts make_consts(const pbf_solver* s) {
     K.cap = s->cap;
     K.n = s->n;
     K.nstride = s->nstride;
+    K.nbuckets = s->nbuckets;
     return K;
 }
 
@@ -200,37 +205,37 @@ int enqueue_step(pbf_solver* s, hipEvent_t* ev) {
     const bool stats = dbg || (s->P.flags & PBF_FLAG_DENSITY_STATS);
     if (ev) HIP_TRY(hipEventRecord(ev[0], st));
     PROF("start");
-    launch_predict_hash(st, K, B.pos[s->pc], B.vel[s->vc], B);
+    launch_predict_hash(st, K, s->geo, B.pos[s->pc], B.vel[s->vc], B);
     PROF("predict_hash");
     if (ev) HIP_TRY(hipEventRecord(ev[1], st));
     launch_scan(st, K, B);
     PROF("scan");
     if (ev) HIP_TRY(hipEventRecord(ev[2], st));
-    launch_sort_reorder(st, K, B, B.pos[s->pc], B.ids[s->ic], B.pos[s->pc ^ 1], B.vel[s->vc ^ 1], B.ids[s->ic ^ 1],
+    launch_sort_reorder(st, K, s->geo, B, B.pos[s->pc], B.ids[s->ic], B.pos[s->pc ^ 1], B.vel[s->vc ^ 1], B.ids[s->ic ^ 1],
                         dbg ? s->dbg_gid : nullptr);
     PROF("scatter_rank_reorder");
     if (ev) HIP_TRY(hipEventRecord(ev[3], st));
     int a = s->pc ^ 1;
     int v = s->vc ^ 1;
     const LearnArgs L0{};
-    launch_neighbors_lambda(st, K, B.pos[a], B, dbg ? s->dbg_lam : nullptr, L0);
+    launch_neighbors_lambda(st, K, s->geo, s->tune, B.pos[a], B, dbg ? s->dbg_lam : nullptr, L0);
     PROF("neighbors_lambda");
     if (ev) HIP_TRY(hipEventRecord(ev[4], st));
     const int I = s->P.iterations;
     for (int it = 0; it < I; ++it) {
         if (it > 0) {
-            launch_lambda(st, K, B.pos[a], B, L0);
+            launch_lambda(st, K, s->tune, B.pos[a], B, L0);
             PROF("lambda");
         }
-        launch_delta_apply(st, K, B.pos[a], B.pos[a ^ 1], B.vel[v], B, it == I - 1, it == 0, L0);
+        launch_delta_apply(st, K, s->geo, s->tune, B.pos[a], B.pos[a ^ 1], B.vel[v], B, it == I - 1, it == 0, L0);
         PROF("delta_apply");
         a ^= 1;
     }
     if (ev) HIP_TRY(hipEventRecord(ev[5], st));
-    launch_curl(st, K, B.pos[a], B.vel[v], B, dbg ? s->dbg_err : nullptr, stats ? s->partial : nullptr);
+    launch_curl(st, K, s->tune, B.pos[a], B.vel[v], B, dbg ? s->dbg_err : nullptr, stats ? s->partial : nullptr);
     PROF("curl");
     if (ev) HIP_TRY(hipEventRecord(ev[6], st));
-    launch_vort_apply(st, K, B.pos[a], B.vel[v], B.vel[v ^ 1], B, nullptr);
+    launch_vort_apply(st, K, s->tune, B.pos[a], B.vel[v], B.vel[v ^ 1], B, nullptr);
     PROF("vort_apply");
     if (ev) HIP_TRY(hipEventRecord(ev[7], st));
     HIP_TRY(hipGetLastError());
@@ -292,15 +297,15 @@ const char* pbf_last_error(void) { return g_last_error.c_str(); }
 
 const char* pbf_version(void) { return "pbf_hip 0.1 gfx950"; }
 
-int pbf_create(const pbf_params* para, const float grid_space[6], const float boundary[6], const float* pos_xyz,
-               const float* vel_xyz, int32_t n, pbf_solver** out) {
-    if (!out) return set_error(PBF_ERR_ARG, "out is NULL");
-    *out = nullptr;
-    int rc = check_params(para);
-    if (rc) return rc;
-    if (n < 0) return set_error(PBF_ERR_ARG, "n < 0");
-    if (n > 0 && !pos_xyz) return set_error(PBF_ERR_ARG, "pos_xyz is NULL");
-    if (!grid_space || !boundary) return set_error(PBF_ERR_ARG, "grid_space / boundary is NULL");
+}  // extern "C"
+
+namespace {
+
+// Shared by pbf_create and pbf_create_batch: allocates every buffer for n particles and
+// nbuckets cell counters and uploads the state.  geoms / sid (host) describe a batch.
+int create_solver(const pbf_params* para, const float* grid_space, const float* boundary, const float* pos_xyz,
+                  const float* vel_xyz, int32_t n, int nbuckets, const std::vector<SceneGeom>* geoms,
+                  const std::vector<uint8_t>* sid, pbf_solver** out) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(PBF_ERR_NOGPU, "no HIP device");
     pbf_solver* s = new pbf_solver();
@@ -310,6 +315,7 @@ int pbf_create(const pbf_params* para, const float grid_space[6], const float bo
     std::memcpy(s->bd, boundary, sizeof(s->bd));
     s->n = n;
     s->ncells = para->grid_dims[0] * para->grid_dims[1] * para->grid_dims[2];
+    s->nbuckets = nbuckets;
     s->nstride = ((n + 63) / 64) * 64;
     if (s->nstride == 0) s->nstride = 64;
     s->cap = para->max_neighbors;
@@ -323,8 +329,9 @@ int pbf_create(const pbf_params* para, const float grid_space[6], const float bo
         return fail(set_error(PBF_ERR_HIP, "hipStreamCreate failed"));
     StepBuffers& B = s->B;
     size_t np = (size_t)std::max(n, 1);
-    int m = s->ncells + 1;
+    int m = nbuckets;
     size_t mpad = (size_t)scan_tiles(m) * 1024;
+    int rc;
     for (int i = 0; i < 2; ++i) {
         if ((rc = dalloc(s, &B.pos[i], np))) return fail(rc);
         if ((rc = dalloc(s, &B.vel[i], np))) return fail(rc);
@@ -336,7 +343,7 @@ int pbf_create(const pbf_params* para, const float grid_space[6], const float bo
         (rc = dalloc(s, &B.tile_sum, (size_t)scan_tiles(m) + 4)) || (rc = dalloc(s, &B.bucket, np)) ||
         (rc = dalloc(s, &B.nbr, (size_t)s->cap * s->nstride)) || (rc = dalloc(s, &B.ncnt, np + 16)))
         return fail(rc);
-    s->npartial = err_partial_blocks((int)np);
+    s->npartial = (int)((np + 63) / 64) + 1;  // enough for the smallest workgroup size
     if ((rc = dalloc(s, &s->feat, np * kNumFeatures)) || (rc = dalloc(s, &s->lam_sum, np)) ||
         (rc = dalloc(s, &s->resid, (size_t)s->npartial)) || (rc = dalloc(s, &s->resid0, (size_t)s->npartial)))
         return fail(rc);
@@ -345,6 +352,17 @@ int pbf_create(const pbf_params* para, const float grid_space[6], const float bo
     if ((rc = dalloc(s, &s->dbg_gid, np)) || (rc = dalloc(s, &s->dbg_lam, np)) || (rc = dalloc(s, &s->dbg_err, np)) ||
         (rc = dalloc(s, &s->partial, (size_t)s->npartial * 3)))
         return fail(rc);
+    if (geoms) {
+        SceneGeom* dg = nullptr;
+        uint8_t* ds = nullptr;
+        if ((rc = dalloc(s, &dg, geoms->size())) || (rc = dalloc(s, &ds, np))) return fail(rc);
+        if (hipMemcpy(dg, geoms->data(), sizeof(SceneGeom) * geoms->size(), hipMemcpyHostToDevice) != hipSuccess ||
+            (n > 0 && hipMemcpy(ds, sid->data(), (size_t)n, hipMemcpyHostToDevice) != hipSuccess))
+            return fail(set_error(PBF_ERR_HIP, "scene table upload failed"));
+        s->geo.geoms = dg;
+        s->geo.sid = ds;
+        s->nscenes = (int)geoms->size();
+    }
     launch_fill_u32(s->stream, B.count, mpad, 0u);
     if (hipGetLastError() != hipSuccess) return fail(set_error(PBF_ERR_HIP, "fill launch failed"));
     *out = s;
@@ -353,6 +371,81 @@ int pbf_create(const pbf_params* para, const float grid_space[6], const float bo
         *out = nullptr;
         return fail(rc);
     }
+    return PBF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pbf_create(const pbf_params* para, const float grid_space[6], const float boundary[6], const float* pos_xyz,
+               const float* vel_xyz, int32_t n, pbf_solver** out) {
+    if (!out) return set_error(PBF_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int rc = check_params(para);
+    if (rc) return rc;
+    if (n < 0) return set_error(PBF_ERR_ARG, "n < 0");
+    if (n > 0 && !pos_xyz) return set_error(PBF_ERR_ARG, "pos_xyz is NULL");
+    if (!grid_space || !boundary) return set_error(PBF_ERR_ARG, "grid_space / boundary is NULL");
+    int ncells = para->grid_dims[0] * para->grid_dims[1] * para->grid_dims[2];
+    return create_solver(para, grid_space, boundary, pos_xyz, vel_xyz, n, ncells + 1, nullptr, nullptr, out);
+}
+
+int pbf_create_batch(const pbf_params* para, int32_t nscenes, const int32_t* counts, const float* grid_space,
+                     const float* boundary, const int32_t* grid_dims, const float* pos_xyz, const float* vel_xyz,
+                     pbf_solver** out) {
+    if (!out) return set_error(PBF_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int rc = check_params(para);
+    if (rc) return rc;
+    if (nscenes < 1 || nscenes > 255 || !counts || !grid_space || !boundary || !grid_dims)
+        return set_error(PBF_ERR_ARG, "bad scene description (1..255 scenes)");
+    std::vector<SceneGeom> geoms(nscenes);
+    int64_t n = 0, nb = 0;
+    for (int k = 0; k < nscenes; ++k) {
+        if (counts[k] < 0) return set_error(PBF_ERR_ARG, "negative scene count");
+        SceneGeom& G = geoms[k];
+        int64_t cells = 1;
+        for (int a = 0; a < 3; ++a) {
+            int d = grid_dims[3 * k + a];
+            if (d < 1 || d > 4096) return set_error(PBF_ERR_ARG, "scene grid dims out of range");
+            G.dims[a] = d;
+            G.gmin[a] = grid_space[6 * k + a];
+            G.bmin[a] = boundary[6 * k + a];
+            G.bmax[a] = boundary[6 * k + 3 + a];
+            cells *= d;
+        }
+        G.coff = (int)nb;
+        G.ncells = (int)cells;
+        nb += cells + 1;
+        n += counts[k];
+        if (nb > ((int64_t)1 << 28) || n > ((int64_t)1 << 30)) return set_error(PBF_ERR_ARG, "batch too large");
+    }
+    if (n > 0 && !pos_xyz) return set_error(PBF_ERR_ARG, "pos_xyz is NULL");
+    std::vector<uint8_t> sid((size_t)n);
+    for (int64_t k = 0, i = 0; k < nscenes; ++k)
+        for (int j = 0; j < counts[k]; ++j) sid[i++] = (uint8_t)k;
+    pbf_params p = *para;
+    for (int a = 0; a < 3; ++a) p.grid_dims[a] = geoms[0].dims[a];
+    return create_solver(&p, grid_space, boundary, pos_xyz, vel_xyz, (int32_t)n, (int)nb, &geoms, &sid, out);
+}
+
+int pbf_set_tuning(pbf_solver* s, int32_t block, int32_t grid, int32_t xcd_map) {
+    if (!s) return set_error(PBF_ERR_ARG, "solver is NULL");
+    if (block != 64 && block != 128 && block != 256 && block != 512) return set_error(PBF_ERR_ARG, "block must be 64/128/256/512");
+    if (grid < 0) return set_error(PBF_ERR_ARG, "grid < 0");
+    s->tune.block = block;
+    s->tune.grid = grid;
+    s->tune.xcd_map = xcd_map ? 1 : 0;
+    drop_graphs(s);
+    return PBF_OK;
+}
+
+int pbf_get_tuning(pbf_solver* s, int32_t out[3]) {
+    if (!s || !out) return set_error(PBF_ERR_ARG, "NULL argument");
+    out[0] = s->tune.block;
+    out[1] = s->tune.grid;
+    out[2] = s->tune.xcd_map;
     return PBF_OK;
 }
 
@@ -477,6 +570,7 @@ int pbf_get_grid(pbf_solver* s, int32_t* gid, int32_t* count, int32_t* cell_star
     if (!s) return set_error(PBF_ERR_ARG, "solver is NULL");
     if (!(s->P.flags & PBF_FLAG_DEBUG) || s->steps_done == 0)
         return set_error(PBF_ERR_STATE, "needs a step taken with PBF_FLAG_DEBUG");
+    if (s->nscenes > 1) return set_error(PBF_ERR_STATE, "pbf_get_grid describes single-scene solvers");
     HIP_TRY(hipSetDevice(s->device));
     HIP_TRY(hipStreamSynchronize(s->stream));
     std::vector<uint32_t> cs(s->ncells + 1);
@@ -549,7 +643,7 @@ int pbf_density_stats(pbf_solver* s, float out[3]) {
     if (!s->stats_valid) return set_error(PBF_ERR_STATE, "no step with density statistics taken");
     HIP_TRY(hipSetDevice(s->device));
     HIP_TRY(hipStreamSynchronize(s->stream));
-    int nb = err_partial_blocks(std::max(s->n, 1));
+    int nb = reduce_blocks(s->tune, s->n);
     std::vector<float> p((size_t)nb * 3);
     HIP_TRY(hipMemcpy(p.data(), s->partial, sizeof(float) * p.size(), hipMemcpyDeviceToHost));
     double sa = 0.0, sp = 0.0;
@@ -624,6 +718,92 @@ int pbf_profile_step(pbf_solver* s, float* ms, const char** names, int32_t max) 
     return rc == PBF_OK ? cnt : rc;
 }
 
+
+// Times every launch shape (workgroup size x persistent grid x XCD mapping) over `steps`
+// graph-replayed steps each and keeps the fastest.  Results do not depend on the launch shape,
+// and the state is restored afterwards, so tuning does not advance the simulation.  best[3]
+// (may be NULL) receives block, grid, xcd_map; best_ms the fastest time per step.
+int pbf_autotune(pbf_solver* s, int32_t steps, int32_t* best, float* best_ms) {
+    if (!s || steps < 1) return set_error(PBF_ERR_ARG, "bad argument");
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const int n = s->n;
+    size_t np = (size_t)std::max(n, 1);
+    float4 *sp = nullptr, *sv = nullptr;
+    uint32_t* si = nullptr;
+    HIP_TRY(hipMalloc(&sp, sizeof(float4) * np));
+    HIP_TRY(hipMalloc(&sv, sizeof(float4) * np));
+    HIP_TRY(hipMalloc(&si, sizeof(uint32_t) * np));
+    HIP_TRY(hipMemcpyAsync(sp, s->B.pos[s->pc], sizeof(float4) * np, hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(sv, s->B.vel[s->vc], sizeof(float4) * np, hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(si, s->B.ids[s->ic], sizeof(uint32_t) * np, hipMemcpyDeviceToDevice, s->stream));
+    const int pc0 = s->pc, vc0 = s->vc, ic0 = s->ic, sd0 = s->steps_done;
+    const Tuning keep = s->tune;
+    const uint32_t flags0 = s->P.flags;
+    s->P.flags = (flags0 | PBF_FLAG_GRAPH) & ~(PBF_FLAG_DEBUG | PBF_FLAG_DENSITY_STATS);
+    int cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, s->device) == hipSuccess && prop.multiProcessorCount > 0)
+        cus = prop.multiProcessorCount;
+    const int blocks[4] = {64, 128, 256, 512};
+    const int grids[4] = {0, 2 * cus, 4 * cus, 8 * cus};
+    Tuning bestT = keep;
+    float bestMs = INFINITY;
+    int rc = PBF_OK;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    for (int bi = 0; bi < 4 && rc == PBF_OK; ++bi)
+        for (int gi = 0; gi < 4 && rc == PBF_OK; ++gi)
+            for (int xm = 0; xm < 2 && rc == PBF_OK; ++xm) {
+                if (grids[gi] > 0 && (int)((n + blocks[bi] - 1) / blocks[bi]) <= grids[gi]) continue;
+                s->tune = Tuning{blocks[bi], grids[gi], xm};
+                drop_graphs(s);
+                for (int w = 0; w < 2 && rc == PBF_OK; ++w) rc = run_step(s);  // capture + warm up
+                if (rc) break;
+                if (hipEventRecord(e0, s->stream) != hipSuccess) { rc = set_error(PBF_ERR_HIP, "event"); break; }
+                for (int k = 0; k < steps && rc == PBF_OK; ++k) rc = run_step(s);
+                if (rc) break;
+                if (hipEventRecord(e1, s->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+                    rc = set_error(PBF_ERR_HIP, "event");
+                    break;
+                }
+                float ms = 0.0f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                ms /= (float)steps;
+                if (ms < bestMs) {
+                    bestMs = ms;
+                    bestT = s->tune;
+                }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // restore the state the tuning started from
+    s->pc = pc0;
+    s->vc = vc0;
+    s->ic = ic0;
+    s->steps_done = sd0;
+    s->P.flags = flags0;
+    drop_graphs(s);
+    hipError_t e = hipMemcpyAsync(s->B.pos[s->pc], sp, sizeof(float4) * np, hipMemcpyDeviceToDevice, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->B.vel[s->vc], sv, sizeof(float4) * np, hipMemcpyDeviceToDevice, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->B.ids[s->ic], si, sizeof(uint32_t) * np, hipMemcpyDeviceToDevice, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(sp);
+    (void)hipFree(sv);
+    (void)hipFree(si);
+    if (e != hipSuccess) return set_error(PBF_ERR_HIP, "state restore failed: %s", hipGetErrorString(e));
+    s->tune = rc == PBF_OK ? bestT : keep;
+    if (rc) return rc;
+    if (best) {
+        best[0] = bestT.block;
+        best[1] = bestT.grid;
+        best[2] = bestT.xcd_map;
+    }
+    if (best_ms) *best_ms = bestMs;
+    return PBF_OK;
+}
+
 }  // extern "C"
 
 // ============================================================================================
@@ -687,7 +867,7 @@ uint64_t sm64(uint64_t& st) {
 
 // Sum of the residual block partials / n (compression residual, mean max(C, 0)).
 int read_residual(pbf_solver* s, const float* partials, float* out) {
-    int nb = err_partial_blocks(std::max(s->n, 1));
+    int nb = reduce_blocks(s->tune, s->n);
     HIP_TRY(hipMemcpyAsync(s->resid_host, partials, sizeof(float) * nb, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     double acc = 0.0;
@@ -964,9 +1144,9 @@ int pbf_step_adaptive(pbf_solver* s, int32_t use_mlp, float tol, int32_t min_ite
     hipStream_t st = s->stream;
     const bool dbg = (s->P.flags & PBF_FLAG_DEBUG) != 0;
     const bool stats = dbg || (s->P.flags & PBF_FLAG_DENSITY_STATS);
-    launch_predict_hash(st, K, B.pos[s->pc], B.vel[s->vc], B);
+    launch_predict_hash(st, K, s->geo, B.pos[s->pc], B.vel[s->vc], B);
     launch_scan(st, K, B);
-    launch_sort_reorder(st, K, B, B.pos[s->pc], B.ids[s->ic], B.pos[s->pc ^ 1], B.vel[s->vc ^ 1], B.ids[s->ic ^ 1],
+    launch_sort_reorder(st, K, s->geo, B, B.pos[s->pc], B.ids[s->ic], B.pos[s->pc ^ 1], B.vel[s->vc ^ 1], B.ids[s->ic ^ 1],
                         dbg ? s->dbg_gid : nullptr);
     int a = s->pc ^ 1;
     const int v = s->vc ^ 1;
@@ -974,7 +1154,7 @@ int pbf_step_adaptive(pbf_solver* s, int32_t use_mlp, float tol, int32_t min_ite
     LF.feat = s->feat;
     LF.vel = B.vel[v];
     LF.resid = use_mlp ? s->resid0 : nullptr;
-    launch_neighbors_lambda(st, K, B.pos[a], B, dbg ? s->dbg_lam : nullptr, LF);
+    launch_neighbors_lambda(st, K, s->geo, s->tune, B.pos[a], B, dbg ? s->dbg_lam : nullptr, LF);
     LearnArgs LA{};
     LA.lam_sum = s->lam_sum;
     LearnArgs LR{};
@@ -984,8 +1164,8 @@ int pbf_step_adaptive(pbf_solver* s, int32_t use_mlp, float tol, int32_t min_ite
     bool done = false;
     if (use_mlp) {
         launch_mlp_infer(st, s->mlp, s->feat, s->n, B.pos[a], nullptr, s->P.h * s->P.h);
-        launch_delta_apply(st, K, B.pos[a], B.pos[a ^ 1], B.vel[v], B, 0, 1, LA);
-        launch_lambda(st, K, B.pos[a ^ 1], B, LR);
+        launch_delta_apply(st, K, s->geo, s->tune, B.pos[a], B.pos[a ^ 1], B.vel[v], B, 0, 1, LA);
+        launch_lambda(st, K, s->tune, B.pos[a ^ 1], B, LR);
         HIP_TRY(hipGetLastError());
         float r0 = 0.0f, rg = 0.0f;
         int rc = read_residual(s, s->resid0, &r0);
@@ -1002,24 +1182,24 @@ int pbf_step_adaptive(pbf_solver* s, int32_t use_mlp, float tol, int32_t min_ite
         } else {
             // rejected: restore lambda of iteration 1 at the predicted positions and run plain
             LearnArgs LN{};
-            launch_neighbors_lambda(st, K, B.pos[a], B, nullptr, LN);
+            launch_neighbors_lambda(st, K, s->geo, s->tune, B.pos[a], B, nullptr, LN);
             s->guess_accepted = 0;
         }
     }
     while (!done) {
-        launch_delta_apply(st, K, B.pos[a], B.pos[a ^ 1], B.vel[v], B, 0, it == 0, LA);
+        launch_delta_apply(st, K, s->geo, s->tune, B.pos[a], B.pos[a ^ 1], B.vel[v], B, 0, it == 0, LA);
         a ^= 1;
         ++it;
         if (it >= max_iters) break;
-        launch_lambda(st, K, B.pos[a], B, LR);
+        launch_lambda(st, K, s->tune, B.pos[a], B, LR);
         HIP_TRY(hipGetLastError());
         int rc = read_residual(s, s->resid, &r);
         if (rc) return rc;
         if (it >= min_iters && r <= tol) break;
     }
     launch_velocity(st, K, B.pos[a], B.vel[v], B);
-    launch_curl(st, K, B.pos[a], B.vel[v], B, dbg ? s->dbg_err : nullptr, stats ? s->partial : nullptr);
-    launch_vort_apply(st, K, B.pos[a], B.vel[v], B.vel[v ^ 1], B, s->lam_sum);
+    launch_curl(st, K, s->tune, B.pos[a], B.vel[v], B, dbg ? s->dbg_err : nullptr, stats ? s->partial : nullptr);
+    launch_vort_apply(st, K, s->tune, B.pos[a], B.vel[v], B.vel[v ^ 1], B, s->lam_sum);
     HIP_TRY(hipGetLastError());
     s->lam_last = a ^ 1;
     s->pc = a;

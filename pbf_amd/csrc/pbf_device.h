@@ -12,6 +12,16 @@
 
 namespace pbf {
 
+// Grid and clamp box of one scene.  A batch concatenates scenes: scene s owns cells
+// [coff, coff + ncells] (the last one collects its out-of-grid particles) and a fixed slot range.
+struct SceneGeom {
+    float gmin[3];
+    float bmin[3], bmax[3];
+    int dims[3];
+    int coff;
+    int ncells;
+};
+
 // Everything a step kernel needs, passed by value as one kernel argument.
 struct StepConsts {
     float h, h2, c6, cs, wdq, k, scorr_n, restRho, eps, dt, visc_c, vort_eps, jitter, rmin;
@@ -25,7 +35,22 @@ struct StepConsts {
     int cap;     // neighbour cap (the reference's MAX_NEIGHBOR = 63)
     int n;       // particles
     int nstride; // row stride of the neighbour table (>= n, multiple of 64)
+    int nbuckets;  // cell counters of the whole step (ncells + 1, or the sum over a batch)
 };
+
+// The geometry of a single-scene solver, from its constants.
+__device__ __forceinline__ SceneGeom geom_single(const StepConsts& K) {
+    SceneGeom G;
+    for (int a = 0; a < 3; ++a) {
+        G.gmin[a] = K.gmin[a];
+        G.bmin[a] = K.bmin[a];
+        G.bmax[a] = K.bmax[a];
+        G.dims[a] = K.dims[a];
+    }
+    G.coff = 0;
+    G.ncells = K.ncells;
+    return G;
+}
 
 // sin for the boundary jitter: Cody & Waite (1980) reduction by pi/2 in three parts
 // (P1: 8 significant bits, P2: 12) and the Taylor series of sin / cos on [-pi/4, pi/4],
@@ -54,13 +79,13 @@ __device__ __forceinline__ float random_offset(float u, float v) {
 }
 
 // BoundParticle, cuh:121-140, including the y-max case taking its jitter from (y, z).
-__device__ __forceinline__ void bound_particle(float& x, float& y, float& z, const StepConsts& K) {
-    if (x < K.bmin[0]) x = K.bmin[0] + K.jitter * random_offset(y, z);
-    if (x > K.bmax[0]) x = K.bmax[0] - K.jitter * random_offset(y, z);
-    if (y < K.bmin[1]) y = K.bmin[1] + K.jitter * random_offset(x, z);
-    if (y > K.bmax[1]) y = K.bmax[1] - K.jitter * random_offset(y, z);
-    if (z < K.bmin[2]) z = K.bmin[2] + K.jitter * random_offset(x, y);
-    if (z > K.bmax[2]) z = K.bmax[2] - K.jitter * random_offset(x, y);
+__device__ __forceinline__ void bound_particle(float& x, float& y, float& z, const StepConsts& K, const SceneGeom& G) {
+    if (x < G.bmin[0]) x = G.bmin[0] + K.jitter * random_offset(y, z);
+    if (x > G.bmax[0]) x = G.bmax[0] - K.jitter * random_offset(y, z);
+    if (y < G.bmin[1]) y = G.bmin[1] + K.jitter * random_offset(x, z);
+    if (y > G.bmax[1]) y = G.bmax[1] - K.jitter * random_offset(y, z);
+    if (z < G.bmin[2]) z = G.bmin[2] + K.jitter * random_offset(x, y);
+    if (z > G.bmax[2]) z = G.bmax[2] - K.jitter * random_offset(x, y);
 }
 
 // Wpoly6, cuh:99-103, on r.r.
@@ -105,13 +130,14 @@ __device__ __forceinline__ int dim_of(float p, float pmin, float invH, int dim) 
     return (int)f;
 }
 
-// computeGid, cuh:172-181; out-of-grid particles go to bucket ncells.
-__device__ __forceinline__ int cell_of(float x, float y, float z, const StepConsts& K) {
-    int d0 = dim_of(x, K.gmin[0], K.invH, K.dims[0]);
-    int d1 = dim_of(y, K.gmin[1], K.invH, K.dims[1]);
-    int d2 = dim_of(z, K.gmin[2], K.invH, K.dims[2]);
-    bool out = d0 < 0 || d0 >= K.dims[0] || d1 < 0 || d1 >= K.dims[1] || d2 < 0 || d2 >= K.dims[2];
-    return out ? K.ncells : d0 + d1 * K.dims[0] + d2 * K.dims[0] * K.dims[1];
+// computeGid, cuh:172-181, offset by the scene's first cell; out-of-grid particles go to the
+// scene's bucket coff + ncells.
+__device__ __forceinline__ int cell_of(float x, float y, float z, const StepConsts& K, const SceneGeom& G) {
+    int d0 = dim_of(x, G.gmin[0], K.invH, G.dims[0]);
+    int d1 = dim_of(y, G.gmin[1], K.invH, G.dims[1]);
+    int d2 = dim_of(z, G.gmin[2], K.invH, G.dims[2]);
+    bool out = d0 < 0 || d0 >= G.dims[0] || d1 < 0 || d1 >= G.dims[1] || d2 < 0 || d2 >= G.dims[2];
+    return G.coff + (out ? G.ncells : d0 + d1 * G.dims[0] + d2 * G.dims[0] * G.dims[1]);
 }
 
 }  // namespace pbf

@@ -43,27 +43,42 @@ struct LearnArgs {
     float* resid;        // block sums of max(C, 0) (lambda passes)
 };
 
+// Per-slot scene geometry of a batch (both NULL for a single scene).
+struct Geometry {
+    const SceneGeom* geoms;
+    const uint8_t* sid;
+};
+
+// Launch shape of the neighbour kernels (results do not depend on it).
+struct Tuning {
+    int block;    // workgroup size: 64, 128, 256 or 512
+    int grid;     // > 0: persistent grid of this many workgroups, else one per block
+    int xcd_map;  // map workgroups to contiguous particle chunks per XCD
+};
+
 int scan_tiles(int m);
-int err_partial_blocks(int n);
+// Workgroups (= reduction partials) of a neighbour-kernel launch over n particles.
+int reduce_blocks(const Tuning& T, int n);
 
 // predict + clamp + hash + count (reads the state, writes xp / vp / cell / local / count)
-void launch_predict_hash(hipStream_t st, const StepConsts& K, const float4* pos, const float4* vel,
+void launch_predict_hash(hipStream_t st, const StepConsts& K, const Geometry& GE, const float4* pos, const float4* vel,
                          const StepBuffers& B);
 // exclusive scan of count into cstart; clears count for the next step
 void launch_scan(hipStream_t st, const StepConsts& K, const StepBuffers& B);
 // scatter into buckets, rank by particle id, move xp / vp / start-of-step position / id to slot order
-void launch_sort_reorder(hipStream_t st, const StepConsts& K, const StepBuffers& B, const float4* pos_in,
-                         const uint32_t* ids_in, float4* pos_out, float4* vel_out, uint32_t* ids_out,
-                         int32_t* dbg_gid);
-void launch_neighbors_lambda(hipStream_t st, const StepConsts& K, float4* pos, const StepBuffers& B, float* dbg_lam,
-                             const LearnArgs& L);
-void launch_lambda(hipStream_t st, const StepConsts& K, float4* pos, const StepBuffers& B, const LearnArgs& L);
-void launch_delta_apply(hipStream_t st, const StepConsts& K, const float4* pos, float4* pos_next, float4* vel,
-                        const StepBuffers& B, int last, int first, const LearnArgs& L);
-void launch_curl(hipStream_t st, const StepConsts& K, const float4* pos, const float4* vel, const StepBuffers& B,
-                 float* err, float* partial);
-void launch_vort_apply(hipStream_t st, const StepConsts& K, const float4* pos, const float4* vel, float4* vel_out,
-                       const StepBuffers& B, const float* lam_sum);
+void launch_sort_reorder(hipStream_t st, const StepConsts& K, const Geometry& GE, const StepBuffers& B,
+                         const float4* pos_in, const uint32_t* ids_in, float4* pos_out, float4* vel_out,
+                         uint32_t* ids_out, int32_t* dbg_gid);
+void launch_neighbors_lambda(hipStream_t st, const StepConsts& K, const Geometry& GE, const Tuning& T, float4* pos,
+                             const StepBuffers& B, float* dbg_lam, const LearnArgs& L);
+void launch_lambda(hipStream_t st, const StepConsts& K, const Tuning& T, float4* pos, const StepBuffers& B,
+                   const LearnArgs& L);
+void launch_delta_apply(hipStream_t st, const StepConsts& K, const Geometry& GE, const Tuning& T, const float4* pos,
+                        float4* pos_next, float4* vel, const StepBuffers& B, int last, int first, const LearnArgs& L);
+void launch_curl(hipStream_t st, const StepConsts& K, const Tuning& T, const float4* pos, const float4* vel,
+                 const StepBuffers& B, float* err, float* partial);
+void launch_vort_apply(hipStream_t st, const StepConsts& K, const Tuning& T, const float4* pos, const float4* vel,
+                       float4* vel_out, const StepBuffers& B, const float* lam_sum);
 void launch_velocity(hipStream_t st, const StepConsts& K, const float4* pos, float4* vel, const StepBuffers& B);
 void launch_export(hipStream_t st, int n, const float4* pos, const float4* vel, const uint32_t* ids, float* pos_xyz,
                    float* vel_xyz);

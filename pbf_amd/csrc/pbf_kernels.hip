@@ -3,14 +3,23 @@
 // Data layout in HBM (all per-particle arrays in "slot" order, the cell-sorted order of the
 // current step; ids[slot] is the caller's particle index):
 //   pos4   float4 (x, y, z, lambda)   double-buffered across Jacobi iterations
-//   vel4   float4 (vx, vy, vz, -)      double-buffered across the vorticity pass
+//   vel4   float4 (vx, vy, vz, M)     double-buffered across the vorticity pass; M = the
+//                                     multiplier accumulated last step (warm start only)
 //   xold4  float4 (x_old)             position at the start of the step
 //   omega4 float4 (wx, wy, wz, |w|)   curl
 //   nbr    uint32 [cap][nstride]       neighbour slots, row j holds every particle's j-th
 //                                     neighbour (coalesced across a wavefront)
 //   ncnt   uint8  [n]                  neighbour count
-//   count / cstart  uint32 [ncells+1] / [ncells+2]  cell histogram and its exclusive scan;
-//                                     bucket ncells collects out-of-grid particles
+//   count / cstart  uint32 [nbuckets] / [nbuckets+1]  cell histogram and its exclusive scan;
+//                                     each scene's last bucket collects its out-of-grid particles
+//
+// A batch of independent scenes is one concatenated particle array: scene s owns cells
+// [coff, coff + ncells] and, because the sort is by global cell, always the same slot range,
+// so one counting sort + scan over all cells is the segmented sort.  geoms[scene_of_slot[s]]
+// gives a slot's grid and clamp box; single-scene solvers pass geoms = NULL and use K.
+//
+// Neighbour kernels are templated on the workgroup size and can run persistent: a grid
+// smaller than the particle blocks makes each workgroup walk a contiguous chunk of blocks.
 //
 // Reference kernels each stage replaces are cited at the kernel.
 
@@ -20,6 +29,38 @@
 namespace pbf {
 
 constexpr int kBlock = 256;
+constexpr int kXcds = 8;  // MI355X: workgroups are dispatched round-robin over 8 XCDs
+
+// Workgroup -> chunk index: the workgroups one XCD receives (blockIdx.x = x, x+8, x+16, ...)
+// take one contiguous range of chunks, so that range's neighbour gathers stay in that XCD's
+// L2.  A bijection on [0, gridDim.x).
+__device__ __forceinline__ int xcd_block() {
+    const int b = blockIdx.x, nb = gridDim.x;
+    const int q = nb / kXcds, r = nb % kXcds, x = b % kXcds;
+    return x * q + min(x, r) + b / kXcds;
+}
+
+// Particle blocks of BS slots this workgroup handles: one (full grid) or a contiguous chunk
+// (persistent grid).
+struct BlockRange {
+    int first, last;
+    __device__ __forceinline__ BlockRange(int nblocks, bool xcd_map) {
+        const int g = xcd_map ? xcd_block() : (int)blockIdx.x;
+        if ((int)gridDim.x >= nblocks) {
+            first = g;
+            last = g + 1;
+        } else {
+            const int per = (nblocks + gridDim.x - 1) / gridDim.x;
+            first = g * per;
+            last = min(nblocks, first + per);
+        }
+    }
+};
+
+__device__ __forceinline__ SceneGeom geom_of(const StepConsts& K, const SceneGeom* __restrict__ geoms,
+                                             const uint8_t* __restrict__ sid, int s) {
+    return geoms ? geoms[sid[s]] : geom_single(K);
+}
 
 // ---------------------------------------------------------------------------------------
 // wavefront (64-lane) inclusive scan with DPP: Hillis-Steele inside each row of 16 lanes
@@ -43,12 +84,14 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
 // ---------------------------------------------------------------------------------------
 // updateParticles (cu:4-13) + calculateParticlesGid (cu:31-40): predict, clamp, hash, count.
 // The returned atomic slot gives an (unordered) place inside the cell bucket.
-__global__ __launch_bounds__(kBlock) void k_predict_hash(StepConsts K, const float4* __restrict__ pos,
+__global__ __launch_bounds__(kBlock) void k_predict_hash(StepConsts K, const SceneGeom* __restrict__ geoms,
+                                                         const uint8_t* __restrict__ sid, const float4* __restrict__ pos,
                                                          const float4* __restrict__ vel, float4* __restrict__ xp,
                                                          float4* __restrict__ vp, uint32_t* __restrict__ cell,
                                                          uint32_t* __restrict__ local, uint32_t* __restrict__ count) {
     int s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= K.n) return;
+    const SceneGeom G = geom_of(K, geoms, sid, s);
     float4 x = pos[s];
     float4 v = vel[s];
     v.x = v.x + K.dt * K.g[0];
@@ -57,8 +100,8 @@ __global__ __launch_bounds__(kBlock) void k_predict_hash(StepConsts K, const flo
     x.x = x.x + K.dt * v.x;
     x.y = x.y + K.dt * v.y;
     x.z = x.z + K.dt * v.z;
-    bound_particle(x.x, x.y, x.z, K);
-    int c = cell_of(x.x, x.y, x.z, K);
+    bound_particle(x.x, x.y, x.z, K, G);
+    int c = cell_of(x.x, x.y, x.z, K, G);
     uint32_t l = atomicAdd(&count[c], 1u);
     xp[s] = make_float4(x.x, x.y, x.z, 0.0f);
     vp[s] = v;  // .w carries the previous step's accumulated multiplier (warm start)
@@ -169,7 +212,9 @@ __global__ __launch_bounds__(kBlock) void k_scatter(int n, const uint32_t* __res
 // Orders each bucket by ascending particle id (rank = members with a smaller id) and moves the
 // particle's predicted position, velocity, start-of-step position and id to its final slot.
 // Out-of-grid particles keep their bucket order: nothing reads their relative order.
-__global__ __launch_bounds__(kBlock) void k_rank_reorder(StepConsts K, const uint32_t* __restrict__ bucket,
+__global__ __launch_bounds__(kBlock) void k_rank_reorder(StepConsts K, const SceneGeom* __restrict__ geoms,
+                                                         const uint8_t* __restrict__ sid,
+                                                         const uint32_t* __restrict__ bucket,
                                                          const uint32_t* __restrict__ cell,
                                                          const uint32_t* __restrict__ cstart,
                                                          const uint32_t* __restrict__ ids_in,
@@ -182,8 +227,10 @@ __global__ __launch_bounds__(kBlock) void k_rank_reorder(StepConsts K, const uin
     uint32_t e = bucket[t];
     uint32_t c = cell[e];
     uint32_t id = ids_in[e];
+    const SceneGeom G = geom_of(K, geoms, sid, (int)e);
+    const bool inside = (int)c != G.coff + G.ncells;
     int d = t;
-    if ((int)c < K.ncells) {
+    if (inside) {
         uint32_t st = cstart[c], en = cstart[c + 1];
         uint32_t rank = 0;
         for (uint32_t q = st; q < en; ++q) rank += ids_in[bucket[q]] < id ? 1u : 0u;
@@ -193,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_rank_reorder(StepConsts K, const uin
     vel_out[d] = vp[e];
     xold_out[d] = pos_in[e];
     ids_out[d] = id;
-    if (dbg_gid) dbg_gid[id] = (int)c < K.ncells ? (int32_t)c : -1;
+    if (dbg_gid) dbg_gid[id] = inside ? (int32_t)c - G.coff : -1;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -218,13 +265,14 @@ struct LambdaAcc {
     }
 };
 
-// Block sum of max(C, 0) (the compression residual) in a fixed tree order.
-__device__ void block_residual(float c, float* __restrict__ partial) {
-    __shared__ float sr[kBlock];
+// Block sum of per-thread values in a fixed tree order into partial[blockIdx.x].
+template <int BS>
+__device__ void block_sum(float c, float* __restrict__ partial) {
+    __shared__ float sr[BS];
     int t = threadIdx.x;
     sr[t] = c;
     __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
+    for (int w = BS / 2; w > 0; w >>= 1) {
         if (t < w) sr[t] = sr[t] + sr[t + w];
         __syncthreads();
     }
@@ -235,90 +283,97 @@ __device__ void block_residual(float c, float* __restrict__ partial) {
 // order (the three x-neighbours of a row are one contiguous slot run), self excluded,
 // r.r <= h*h, at most cap hits.  Lambda of iteration 1 goes to pos.w.  With L.feat the raw
 // warm-start features of the particle are written as well (kNumFeatures floats, see
-// pbf_kernels.h); with L.resid the block sums of max(C, 0) at the predicted positions.
-__global__ __launch_bounds__(kBlock) void k_neighbors_lambda(StepConsts K, float4* __restrict__ pos,
-                                                             const uint32_t* __restrict__ cstart,
-                                                             uint32_t* __restrict__ nbr, uint8_t* __restrict__ ncnt,
-                                                             float* __restrict__ dbg_lam, LearnArgs L) {
-    int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= K.n) {
-        if (L.resid) block_residual(0.0f, L.resid);
-        return;
-    }
-    const float4 xi = pos[s];
-    int d0 = dim_of(xi.x, K.gmin[0], K.invH, K.dims[0]);
-    int d1 = dim_of(xi.y, K.gmin[1], K.invH, K.dims[1]);
-    int d2 = dim_of(xi.z, K.gmin[2], K.invH, K.dims[2]);
-    int xlo = max(d0 - 1, 0), xhi = min(d0 + 1, K.dims[0] - 1);
-    LambdaAcc A;
-    A.init();
-    int cnt = 0;
-    uint32_t* out = nbr + s;
-    for (int dz = -1; dz <= 1; ++dz) {
-        int cz = d2 + dz;
-        if (cz < 0 || cz >= K.dims[2]) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            int cy = d1 + dy;
-            if (cy < 0 || cy >= K.dims[1] || xlo > xhi) continue;
-            int row = (cz * K.dims[1] + cy) * K.dims[0];
-            uint32_t b = cstart[row + xlo], e = cstart[row + xhi + 1];
-            for (uint32_t q = b; q < e && cnt < K.cap; ++q) {
-                if ((int)q == s) continue;
-                float4 xj = pos[q];
-                float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-                float r2 = rx * rx + ry * ry + rz * rz;
-                if (r2 <= K.h2) {
-                    out[(size_t)cnt * K.nstride] = q;
-                    ++cnt;
-                    A.add(rx, ry, rz, r2, K);
+// pbf_kernels.h); with L.resid the workgroup sums of max(C, 0) at the predicted positions.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_neighbors_lambda(StepConsts K, const SceneGeom* __restrict__ geoms,
+                                                         const uint8_t* __restrict__ sid, float4* __restrict__ pos,
+                                                         const uint32_t* __restrict__ cstart,
+                                                         uint32_t* __restrict__ nbr, uint8_t* __restrict__ ncnt,
+                                                         float* __restrict__ dbg_lam, LearnArgs L, int xcd_map) {
+    const BlockRange R((K.n + BS - 1) / BS, xcd_map);
+    float comp = 0.0f;
+    for (int blk = R.first; blk < R.last; ++blk) {
+        const int s = blk * BS + threadIdx.x;
+        if (s >= K.n) continue;
+        const SceneGeom G = geom_of(K, geoms, sid, s);
+        const float4 xi = pos[s];
+        int d0 = dim_of(xi.x, G.gmin[0], K.invH, G.dims[0]);
+        int d1 = dim_of(xi.y, G.gmin[1], K.invH, G.dims[1]);
+        int d2 = dim_of(xi.z, G.gmin[2], K.invH, G.dims[2]);
+        int xlo = max(d0 - 1, 0), xhi = min(d0 + 1, G.dims[0] - 1);
+        LambdaAcc A;
+        A.init();
+        int cnt = 0;
+        uint32_t* out = nbr + s;
+        for (int dz = -1; dz <= 1; ++dz) {
+            int cz = d2 + dz;
+            if (cz < 0 || cz >= G.dims[2]) continue;
+            for (int dy = -1; dy <= 1; ++dy) {
+                int cy = d1 + dy;
+                if (cy < 0 || cy >= G.dims[1] || xlo > xhi) continue;
+                int row = G.coff + (cz * G.dims[1] + cy) * G.dims[0];
+                uint32_t b = cstart[row + xlo], e = cstart[row + xhi + 1];
+                for (uint32_t q = b; q < e && cnt < K.cap; ++q) {
+                    if ((int)q == s) continue;
+                    float4 xj = pos[q];
+                    float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+                    float r2 = rx * rx + ry * ry + rz * rz;
+                    if (r2 <= K.h2) {
+                        out[(size_t)cnt * K.nstride] = q;
+                        ++cnt;
+                        A.add(rx, ry, rz, r2, K);
+                    }
                 }
             }
         }
-    }
-    ncnt[s] = (uint8_t)cnt;
-    float lam = A.lambda(K);
-    reinterpret_cast<float*>(pos + s)[3] = lam;
-    if (dbg_lam) dbg_lam[s] = lam;
-    if (L.feat) {
-        const float4 v = L.vel[s];
-        const float ih = 1.0f / K.h, ih2 = ih * ih;
-        float f[kNumFeatures];
-        f[0] = A.rho / K.restRho - 1.0f;
-        f[1] = (float)cnt;
-        f[2] = A.gms / (K.restRho * K.restRho);
-        f[3] = sqrtf(A.gx * A.gx + A.gy * A.gy + A.gz * A.gz) / K.restRho;
-        f[4] = v.x * K.dt * ih;
-        f[5] = v.y * K.dt * ih;
-        f[6] = v.z * K.dt * ih;
-        f[7] = v.w * ih2;
-        float dmin = 4.0f;
-        const float p3[3] = {xi.x, xi.y, xi.z};
-        for (int a = 0; a < 3; ++a) {
-            float lo = fminf(fmaxf((p3[a] - K.bmin[a]) * ih, 0.0f), 4.0f);
-            float hi = fminf(fmaxf((K.bmax[a] - p3[a]) * ih, 0.0f), 4.0f);
-            f[9 + 2 * a] = lo;
-            f[10 + 2 * a] = hi;
-            dmin = fminf(dmin, fminf(lo, hi));
+        ncnt[s] = (uint8_t)cnt;
+        float lam = A.lambda(K);
+        reinterpret_cast<float*>(pos + s)[3] = lam;
+        if (dbg_lam) dbg_lam[s] = lam;
+        if (L.feat) {
+            const float4 v = L.vel[s];
+            const float ih = 1.0f / K.h, ih2 = ih * ih;
+            float f[kNumFeatures];
+            f[0] = A.rho / K.restRho - 1.0f;
+            f[1] = (float)cnt;
+            f[2] = A.gms / (K.restRho * K.restRho);
+            f[3] = sqrtf(A.gx * A.gx + A.gy * A.gy + A.gz * A.gz) / K.restRho;
+            f[4] = v.x * K.dt * ih;
+            f[5] = v.y * K.dt * ih;
+            f[6] = v.z * K.dt * ih;
+            f[7] = v.w * ih2;
+            float dmin = 4.0f;
+            const float p3[3] = {xi.x, xi.y, xi.z};
+            for (int a = 0; a < 3; ++a) {
+                float lo = fminf(fmaxf((p3[a] - G.bmin[a]) * ih, 0.0f), 4.0f);
+                float hi = fminf(fmaxf((G.bmax[a] - p3[a]) * ih, 0.0f), 4.0f);
+                f[9 + 2 * a] = lo;
+                f[10 + 2 * a] = hi;
+                dmin = fminf(dmin, fminf(lo, hi));
+            }
+            f[8] = dmin;
+            f[15] = lam * ih2;
+            float4* o = reinterpret_cast<float4*>(L.feat + (size_t)s * kNumFeatures);
+            o[0] = make_float4(f[0], f[1], f[2], f[3]);
+            o[1] = make_float4(f[4], f[5], f[6], f[7]);
+            o[2] = make_float4(f[8], f[9], f[10], f[11]);
+            o[3] = make_float4(f[12], f[13], f[14], f[15]);
         }
-        f[8] = dmin;
-        f[15] = lam * ih2;
-        float4* o = reinterpret_cast<float4*>(L.feat + (size_t)s * kNumFeatures);
-        o[0] = make_float4(f[0], f[1], f[2], f[3]);
-        o[1] = make_float4(f[4], f[5], f[6], f[7]);
-        o[2] = make_float4(f[8], f[9], f[10], f[11]);
-        o[3] = make_float4(f[12], f[13], f[14], f[15]);
+        comp = comp + fmaxf(A.rho / K.restRho - 1.0f, 0.0f);
     }
-    if (L.resid) block_residual(fmaxf(A.rho / K.restRho - 1.0f, 0.0f), L.resid);
+    if (L.resid) block_sum<BS>(comp, L.resid);
 }
 
-// calculateLambda (cu:156-180) over the frozen list, iterations 2..I.  With L.resid, the block
-// sums of max(C_i, 0) at the current positions are written as well.
-__global__ __launch_bounds__(kBlock) void k_lambda(StepConsts K, float4* __restrict__ pos,
-                                                   const uint32_t* __restrict__ nbr,
-                                                   const uint8_t* __restrict__ ncnt, LearnArgs L) {
-    int s = blockIdx.x * kBlock + threadIdx.x;
+// calculateLambda (cu:156-180) over the frozen list, iterations 2..I.  With L.resid, the
+// workgroup sums of max(C_i, 0) at the current positions are written as well.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_lambda(StepConsts K, float4* __restrict__ pos, const uint32_t* __restrict__ nbr,
+                                               const uint8_t* __restrict__ ncnt, LearnArgs L, int xcd_map) {
+    const BlockRange R((K.n + BS - 1) / BS, xcd_map);
     float comp = 0.0f;
-    if (s < K.n) {
+    for (int blk = R.first; blk < R.last; ++blk) {
+        const int s = blk * BS + threadIdx.x;
+        if (s >= K.n) continue;
         const float4 xi = pos[s];
         int cnt = ncnt[s];
         LambdaAcc A;
@@ -331,90 +386,73 @@ __global__ __launch_bounds__(kBlock) void k_lambda(StepConsts K, float4* __restr
             A.add(rx, ry, rz, r2, K);
         }
         reinterpret_cast<float*>(pos + s)[3] = A.lambda(K);
-        comp = fmaxf(A.rho / K.restRho - 1.0f, 0.0f);
+        comp = comp + fmaxf(A.rho / K.restRho - 1.0f, 0.0f);
     }
-    if (L.resid) block_residual(comp, L.resid);
+    if (L.resid) block_sum<BS>(comp, L.resid);
 }
 
 // calculateOffset (cu:182-199) + updatePosition (cu:15-22): delta-p with s_corr, then
 // x + delta-p clamped into the next position buffer; the last iteration also writes
-// v = (x - x_old) / dt.
-__global__ __launch_bounds__(kBlock) void k_delta_apply(StepConsts K, const float4* __restrict__ pos,
-                                                        float4* __restrict__ pos_next,
-                                                        const uint32_t* __restrict__ nbr,
-                                                        const uint8_t* __restrict__ ncnt,
-                                                        const float4* __restrict__ xold, float4* __restrict__ vel,
-                                                        int last, int first, LearnArgs LA) {
-    int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= K.n) return;
-    const float4 xi = pos[s];
-    int cnt = ncnt[s];
-    if (LA.lam_sum) LA.lam_sum[s] = (first ? 0.0f : LA.lam_sum[s]) + xi.w;
-    float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    const uint32_t* L = nbr + s;
-    for (int j = 0; j < cnt; ++j) {
-        float4 xj = pos[L[(size_t)j * K.nstride]];
-        float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-        float r2 = rx * rx + ry * ry + rz * rz;
-        float w = poly6(r2, K);
-        float t = spiky_scale(r2, K);
-        float a = (xi.w + xj.w) + scorr(w, K);
-        ax = ax + a * (rx * t);
-        ay = ay + a * (ry * t);
-        az = az + a * (rz * t);
-    }
-    float x = xi.x + ax / K.restRho;
-    float y = xi.y + ay / K.restRho;
-    float z = xi.z + az / K.restRho;
-    bound_particle(x, y, z, K);
-    pos_next[s] = make_float4(x, y, z, 0.0f);
-    if (last) {
-        float4 o = xold[s];
-        vel[s] = make_float4((x - o.x) / K.dt, (y - o.y) / K.dt, (z - o.z) / K.dt, 0.0f);
-    }
-}
-
-// Block reduction of the density-error statistics (sum |e|, sum max(e,0), max e) in a fixed
-// tree order.
-__device__ void block_err_stats(float e, bool valid, float* __restrict__ partial) {
-    __shared__ float sa[kBlock], sp[kBlock], sm[kBlock];
-    int t = threadIdx.x;
-    sa[t] = valid ? fabsf(e) : 0.0f;
-    sp[t] = valid ? fmaxf(e, 0.0f) : 0.0f;
-    sm[t] = valid ? e : -INFINITY;
-    __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if (t < w) {
-            sa[t] = sa[t] + sa[t + w];
-            sp[t] = sp[t] + sp[t + w];
-            sm[t] = fmaxf(sm[t], sm[t + w]);
+// v = (x - x_old) / dt.  With LA.lam_sum the multiplier of this iteration is accumulated.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_delta_apply(StepConsts K, const SceneGeom* __restrict__ geoms,
+                                                    const uint8_t* __restrict__ sid, const float4* __restrict__ pos,
+                                                    float4* __restrict__ pos_next, const uint32_t* __restrict__ nbr,
+                                                    const uint8_t* __restrict__ ncnt, const float4* __restrict__ xold,
+                                                    float4* __restrict__ vel, int last, int first, LearnArgs LA,
+                                                    int xcd_map) {
+    const BlockRange R((K.n + BS - 1) / BS, xcd_map);
+    for (int blk = R.first; blk < R.last; ++blk) {
+        const int s = blk * BS + threadIdx.x;
+        if (s >= K.n) continue;
+        const float4 xi = pos[s];
+        int cnt = ncnt[s];
+        if (LA.lam_sum) LA.lam_sum[s] = (first ? 0.0f : LA.lam_sum[s]) + xi.w;
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        const uint32_t* Lst = nbr + s;
+        for (int j = 0; j < cnt; ++j) {
+            float4 xj = pos[Lst[(size_t)j * K.nstride]];
+            float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+            float r2 = rx * rx + ry * ry + rz * rz;
+            float w = poly6(r2, K);
+            float t = spiky_scale(r2, K);
+            float a = (xi.w + xj.w) + scorr(w, K);
+            ax = ax + a * (rx * t);
+            ay = ay + a * (ry * t);
+            az = az + a * (rz * t);
         }
-        __syncthreads();
-    }
-    if (t == 0) {
-        partial[blockIdx.x * 3 + 0] = sa[0];
-        partial[blockIdx.x * 3 + 1] = sp[0];
-        partial[blockIdx.x * 3 + 2] = sm[0];
+        float x = xi.x + ax / K.restRho;
+        float y = xi.y + ay / K.restRho;
+        float z = xi.z + az / K.restRho;
+        const SceneGeom G = geom_of(K, geoms, sid, s);
+        bound_particle(x, y, z, K, G);
+        pos_next[s] = make_float4(x, y, z, 0.0f);
+        if (last) {
+            float4 o = xold[s];
+            vel[s] = make_float4((x - o.x) / K.dt, (y - o.y) / K.dt, (z - o.z) / K.dt, 0.0f);
+        }
     }
 }
 
 // calculateCurl (cu:201-215), plus the density error rho/rho0 - 1 at the final positions when
-// err or partial is given.
-__global__ __launch_bounds__(kBlock) void k_curl(StepConsts K, const float4* __restrict__ pos,
-                                                 const float4* __restrict__ vel, const uint32_t* __restrict__ nbr,
-                                                 const uint8_t* __restrict__ ncnt, float4* __restrict__ omega,
-                                                 float* __restrict__ err, float* __restrict__ partial) {
-    int s = blockIdx.x * kBlock + threadIdx.x;
-    bool valid = s < K.n;
-    float e = 0.0f;
-    if (valid) {
+// err or partial is given (partial: per workgroup sum |e|, sum max(e,0), max e).
+template <int BS>
+__global__ __launch_bounds__(BS) void k_curl(StepConsts K, const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                             const uint32_t* __restrict__ nbr, const uint8_t* __restrict__ ncnt,
+                                             float4* __restrict__ omega, float* __restrict__ err,
+                                             float* __restrict__ partial, int xcd_map) {
+    const BlockRange R((K.n + BS - 1) / BS, xcd_map);
+    float sa = 0.0f, sp = 0.0f, sm = -INFINITY;
+    for (int blk = R.first; blk < R.last; ++blk) {
+        const int s = blk * BS + threadIdx.x;
+        if (s >= K.n) continue;
         const float4 xi = pos[s];
         const float4 vi = vel[s];
         int cnt = ncnt[s];
         float wx = 0.0f, wy = 0.0f, wz = 0.0f, rho = 0.0f;
-        const uint32_t* L = nbr + s;
+        const uint32_t* Lst = nbr + s;
         for (int j = 0; j < cnt; ++j) {
-            uint32_t q = L[(size_t)j * K.nstride];
+            uint32_t q = Lst[(size_t)j * K.nstride];
             float4 xj = pos[q];
             float4 vj = vel[q];
             float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
@@ -428,57 +466,83 @@ __global__ __launch_bounds__(kBlock) void k_curl(StepConsts K, const float4* __r
             rho = rho + poly6(r2, K);
         }
         omega[s] = make_float4(wx, wy, wz, sqrtf(wx * wx + wy * wy + wz * wz));
-        e = rho / K.restRho - 1.0f;
+        float e = rho / K.restRho - 1.0f;
         if (err) err[s] = e;
+        sa = sa + fabsf(e);
+        sp = sp + fmaxf(e, 0.0f);
+        sm = fmaxf(sm, e);
     }
-    if (partial) block_err_stats(e, valid, partial);
+    if (partial) {
+        __shared__ float ra[BS], rp[BS], rm[BS];
+        int t = threadIdx.x;
+        ra[t] = sa;
+        rp[t] = sp;
+        rm[t] = sm;
+        __syncthreads();
+        for (int w = BS / 2; w > 0; w >>= 1) {
+            if (t < w) {
+                ra[t] = ra[t] + ra[t + w];
+                rp[t] = rp[t] + rp[t + w];
+                rm[t] = fmaxf(rm[t], rm[t + w]);
+            }
+            __syncthreads();
+        }
+        if (t == 0) {
+            partial[blockIdx.x * 3 + 0] = ra[0];
+            partial[blockIdx.x * 3 + 1] = rp[0];
+            partial[blockIdx.x * 3 + 2] = rm[0];
+        }
+    }
 }
 
 // calculateVorticityAndViscosity (cu:217-239) + applyVorticityAndViscosity (cu:24-29).
-__global__ __launch_bounds__(kBlock) void k_vort_apply(StepConsts K, const float4* __restrict__ pos,
-                                                       const float4* __restrict__ vel,
-                                                       const float4* __restrict__ omega,
-                                                       const uint32_t* __restrict__ nbr,
-                                                       const uint8_t* __restrict__ ncnt, float4* __restrict__ vel_out,
-                                                       const float* __restrict__ lam_sum) {
-    int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= K.n) return;
-    const float4 xi = pos[s];
-    const float4 vi = vel[s];
-    const float4 wi = omega[s];
-    int cnt = ncnt[s];
-    float px = 0.0f, py = 0.0f, pz = 0.0f, ex = 0.0f, ey = 0.0f, ez = 0.0f;
-    const uint32_t* L = nbr + s;
-    for (int j = 0; j < cnt; ++j) {
-        uint32_t q = L[(size_t)j * K.nstride];
-        float4 xj = pos[q];
-        float4 vj = vel[q];
-        float wl = omega[q].w;
-        float ux = vi.x - vj.x, uy = vi.y - vj.y, uz = vi.z - vj.z;
-        float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-        float r2 = rx * rx + ry * ry + rz * rz;
-        float w = poly6(r2, K);
-        px = px - ux * w;
-        py = py - uy * w;
-        pz = pz - uz * w;
-        float t = spiky_scale(r2, K);
-        ex = ex + wl * (rx * t);
-        ey = ey + wl * (ry * t);
-        ez = ez + wl * (rz * t);
+template <int BS>
+__global__ __launch_bounds__(BS) void k_vort_apply(StepConsts K, const float4* __restrict__ pos,
+                                                   const float4* __restrict__ vel, const float4* __restrict__ omega,
+                                                   const uint32_t* __restrict__ nbr, const uint8_t* __restrict__ ncnt,
+                                                   float4* __restrict__ vel_out, const float* __restrict__ lam_sum,
+                                                   int xcd_map) {
+    const BlockRange R((K.n + BS - 1) / BS, xcd_map);
+    for (int blk = R.first; blk < R.last; ++blk) {
+        const int s = blk * BS + threadIdx.x;
+        if (s >= K.n) continue;
+        const float4 xi = pos[s];
+        const float4 vi = vel[s];
+        const float4 wi = omega[s];
+        int cnt = ncnt[s];
+        float px = 0.0f, py = 0.0f, pz = 0.0f, ex = 0.0f, ey = 0.0f, ez = 0.0f;
+        const uint32_t* Lst = nbr + s;
+        for (int j = 0; j < cnt; ++j) {
+            uint32_t q = Lst[(size_t)j * K.nstride];
+            float4 xj = pos[q];
+            float4 vj = vel[q];
+            float wl = omega[q].w;
+            float ux = vi.x - vj.x, uy = vi.y - vj.y, uz = vi.z - vj.z;
+            float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+            float r2 = rx * rx + ry * ry + rz * rz;
+            float w = poly6(r2, K);
+            px = px - ux * w;
+            py = py - uy * w;
+            pz = pz - uz * w;
+            float t = spiky_scale(r2, K);
+            ex = ex + wl * (rx * t);
+            ey = ey + wl * (ry * t);
+            ez = ez + wl * (rz * t);
+        }
+        float el = sqrtf(ex * ex + ey * ey + ez * ez);
+        if ((double)fabsf(el) > 1e-3) {
+            ex = ex / el;
+            ey = ey / el;
+            ez = ez / el;
+        }
+        float cx = ey * wi.z - ez * wi.y;
+        float cy = ez * wi.x - ex * wi.z;
+        float cz = ex * wi.y - ey * wi.x;
+        float fx = K.vort_eps * cx, fy = K.vort_eps * cy, fz = K.vort_eps * cz;
+        float vx = K.visc_c * px, vy = K.visc_c * py, vz = K.visc_c * pz;
+        vel_out[s] = make_float4(vi.x + (K.dt * fx + vx), vi.y + (K.dt * fy + vy), vi.z + (K.dt * fz + vz),
+                                 lam_sum ? lam_sum[s] : 0.0f);
     }
-    float el = sqrtf(ex * ex + ey * ey + ez * ez);
-    if ((double)fabsf(el) > 1e-3) {
-        ex = ex / el;
-        ey = ey / el;
-        ez = ez / el;
-    }
-    float cx = ey * wi.z - ez * wi.y;
-    float cy = ez * wi.x - ex * wi.z;
-    float cz = ex * wi.y - ey * wi.x;
-    float fx = K.vort_eps * cx, fy = K.vort_eps * cy, fz = K.vort_eps * cz;
-    float vx = K.visc_c * px, vy = K.visc_c * py, vz = K.visc_c * pz;
-    vel_out[s] = make_float4(vi.x + (K.dt * fx + vx), vi.y + (K.dt * fy + vy), vi.z + (K.dt * fz + vz),
-                             lam_sum ? lam_sum[s] : 0.0f);
 }
 
 // updatePosition's velocity (cu:21) for a constraint loop that stopped early.
@@ -529,72 +593,89 @@ __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ p, s
 
 // ---------------------------------------------------------------------------------------
 // launch wrappers
-static inline unsigned nblk(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+static inline unsigned nblk(size_t n, int bs = kBlock) { return (unsigned)((n + bs - 1) / bs); }
 
 int scan_tiles(int m) { return (m + kScanTile - 1) / kScanTile; }
 
-void launch_predict_hash(hipStream_t st, const StepConsts& K, const float4* pos, const float4* vel,
+// Workgroups of a neighbour-kernel launch: one per BS particles, or the persistent grid size.
+static inline unsigned ngrid(const Tuning& T, int n) {
+    unsigned full = nblk(n, T.block);
+    return (T.grid > 0 && (unsigned)T.grid < full) ? (unsigned)T.grid : full;
+}
+
+int reduce_blocks(const Tuning& T, int n) { return (int)ngrid(T, n > 0 ? n : 1); }
+
+void launch_predict_hash(hipStream_t st, const StepConsts& K, const Geometry& GE, const float4* pos, const float4* vel,
                          const StepBuffers& B) {
     if (K.n > 0)
-        hipLaunchKernelGGL(k_predict_hash, dim3(nblk(K.n)), dim3(kBlock), 0, st, K, pos, vel, B.xp, B.vp, B.cell,
-                           B.local, B.count);
+        hipLaunchKernelGGL(k_predict_hash, dim3(nblk(K.n)), dim3(kBlock), 0, st, K, GE.geoms, GE.sid, pos, vel, B.xp,
+                           B.vp, B.cell, B.local, B.count);
 }
 
 void launch_scan(hipStream_t st, const StepConsts& K, const StepBuffers& B) {
-    int m = K.ncells + 1;
+    int m = K.nbuckets;
     int nt = scan_tiles(m);
     hipLaunchKernelGGL(k_scan_reduce, dim3(nt), dim3(kBlock), 0, st, B.count, m, B.tile_sum);
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, st, B.tile_sum, nt, B.cstart, m);
     hipLaunchKernelGGL(k_scan_down, dim3(nt), dim3(kBlock), 0, st, B.count, m, B.tile_sum, B.cstart);
 }
 
-void launch_sort_reorder(hipStream_t st, const StepConsts& K, const StepBuffers& B, const float4* pos_in,
-                         const uint32_t* ids_in, float4* pos_out, float4* vel_out, uint32_t* ids_out,
-                         int32_t* dbg_gid) {
+void launch_sort_reorder(hipStream_t st, const StepConsts& K, const Geometry& GE, const StepBuffers& B,
+                         const float4* pos_in, const uint32_t* ids_in, float4* pos_out, float4* vel_out,
+                         uint32_t* ids_out, int32_t* dbg_gid) {
     int n = K.n;
     if (n <= 0) return;
     hipLaunchKernelGGL(k_scatter, dim3(nblk(n)), dim3(kBlock), 0, st, n, B.cell, B.local, B.cstart, B.bucket);
-    hipLaunchKernelGGL(k_rank_reorder, dim3(nblk(n)), dim3(kBlock), 0, st, K, B.bucket, B.cell, B.cstart, ids_in,
-                       B.xp, B.vp, pos_in, pos_out, vel_out, B.xold, ids_out, dbg_gid);
+    hipLaunchKernelGGL(k_rank_reorder, dim3(nblk(n)), dim3(kBlock), 0, st, K, GE.geoms, GE.sid, B.bucket, B.cell,
+                       B.cstart, ids_in, B.xp, B.vp, pos_in, pos_out, vel_out, B.xold, ids_out, dbg_gid);
 }
 
-void launch_neighbors_lambda(hipStream_t st, const StepConsts& K, float4* pos, const StepBuffers& B, float* dbg_lam,
-                             const LearnArgs& L) {
-    if (K.n > 0)
-        hipLaunchKernelGGL(k_neighbors_lambda, dim3(nblk(K.n)), dim3(kBlock), 0, st, K, pos, B.cstart, B.nbr, B.ncnt,
-                           dbg_lam, L);
+#define PBF_DISPATCH_BS(T, CALL)            \
+    switch ((T).block) {                    \
+        case 64: { constexpr int BS = 64; CALL; } break;   \
+        case 128: { constexpr int BS = 128; CALL; } break; \
+        case 512: { constexpr int BS = 512; CALL; } break; \
+        default: { constexpr int BS = 256; CALL; } break;  \
+    }
+
+void launch_neighbors_lambda(hipStream_t st, const StepConsts& K, const Geometry& GE, const Tuning& T, float4* pos,
+                             const StepBuffers& B, float* dbg_lam, const LearnArgs& L) {
+    if (K.n <= 0) return;
+    PBF_DISPATCH_BS(T, hipLaunchKernelGGL(k_neighbors_lambda<BS>, dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
+                                          GE.sid, pos, B.cstart, B.nbr, B.ncnt, dbg_lam, L, T.xcd_map));
 }
 
-void launch_lambda(hipStream_t st, const StepConsts& K, float4* pos, const StepBuffers& B, const LearnArgs& L) {
-    if (K.n > 0) hipLaunchKernelGGL(k_lambda, dim3(nblk(K.n)), dim3(kBlock), 0, st, K, pos, B.nbr, B.ncnt, L);
+void launch_lambda(hipStream_t st, const StepConsts& K, const Tuning& T, float4* pos, const StepBuffers& B,
+                   const LearnArgs& L) {
+    if (K.n <= 0) return;
+    PBF_DISPATCH_BS(T, hipLaunchKernelGGL(k_lambda<BS>, dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, B.nbr, B.ncnt, L,
+                                          T.xcd_map));
 }
 
-void launch_delta_apply(hipStream_t st, const StepConsts& K, const float4* pos, float4* pos_next, float4* vel,
-                        const StepBuffers& B, int last, int first, const LearnArgs& L) {
-    if (K.n > 0)
-        hipLaunchKernelGGL(k_delta_apply, dim3(nblk(K.n)), dim3(kBlock), 0, st, K, pos, pos_next, B.nbr, B.ncnt,
-                           B.xold, vel, last, first, L);
+void launch_delta_apply(hipStream_t st, const StepConsts& K, const Geometry& GE, const Tuning& T, const float4* pos,
+                        float4* pos_next, float4* vel, const StepBuffers& B, int last, int first, const LearnArgs& L) {
+    if (K.n <= 0) return;
+    PBF_DISPATCH_BS(T, hipLaunchKernelGGL(k_delta_apply<BS>, dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
+                                          GE.sid, pos, pos_next, B.nbr, B.ncnt, B.xold, vel, last, first, L, T.xcd_map));
 }
 
-void launch_curl(hipStream_t st, const StepConsts& K, const float4* pos, const float4* vel, const StepBuffers& B,
-                 float* err, float* partial) {
-    if (K.n > 0)
-        hipLaunchKernelGGL(k_curl, dim3(nblk(K.n)), dim3(kBlock), 0, st, K, pos, vel, B.nbr, B.ncnt, B.omega, err,
-                           partial);
+void launch_curl(hipStream_t st, const StepConsts& K, const Tuning& T, const float4* pos, const float4* vel,
+                 const StepBuffers& B, float* err, float* partial) {
+    if (K.n <= 0) return;
+    PBF_DISPATCH_BS(T, hipLaunchKernelGGL(k_curl<BS>, dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, vel, B.nbr, B.ncnt,
+                                          B.omega, err, partial, T.xcd_map));
 }
 
-void launch_vort_apply(hipStream_t st, const StepConsts& K, const float4* pos, const float4* vel, float4* vel_out,
-                       const StepBuffers& B, const float* lam_sum) {
-    if (K.n > 0)
-        hipLaunchKernelGGL(k_vort_apply, dim3(nblk(K.n)), dim3(kBlock), 0, st, K, pos, vel, B.omega, B.nbr, B.ncnt,
-                           vel_out, lam_sum);
+void launch_vort_apply(hipStream_t st, const StepConsts& K, const Tuning& T, const float4* pos, const float4* vel,
+                       float4* vel_out, const StepBuffers& B, const float* lam_sum) {
+    if (K.n <= 0) return;
+    PBF_DISPATCH_BS(T, hipLaunchKernelGGL(k_vort_apply<BS>, dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, vel, B.omega,
+                                          B.nbr, B.ncnt, vel_out, lam_sum, T.xcd_map));
 }
 
 void launch_velocity(hipStream_t st, const StepConsts& K, const float4* pos, float4* vel, const StepBuffers& B) {
     if (K.n > 0) hipLaunchKernelGGL(k_velocity, dim3(nblk(K.n)), dim3(kBlock), 0, st, K, pos, B.xold, vel);
 }
-
-int err_partial_blocks(int n) { return (int)nblk(n); }
 
 void launch_export(hipStream_t st, int n, const float4* pos, const float4* vel, const uint32_t* ids, float* pos_xyz,
                    float* vel_xyz) {

@@ -218,6 +218,12 @@ def scaled_dam_break(n: int, seed: int, fill=0.25, spacing=0.05, **over) -> Scen
     return _make(f"scaled_dam_break_{n}", full, box, p)
 
 
+def batch_config(seed=3, count=24, lo=8192, hi=262144) -> list:
+    """``count`` dam breaks with log-uniform particle counts in [lo, hi], each in its own box at
+    25% fill (seeds seed*1000 + k)."""
+    return [scaled_dam_break(nk, seed * 1000 + k, fill=0.25) for k, nk in enumerate(batch_sizes(count, lo, hi, seed))]
+
+
 CONFIGS = {
     "cpu_32k": config_cpu_32k,
     "dam_break_262k": config_262k,
