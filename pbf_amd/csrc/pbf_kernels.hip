@@ -265,6 +265,29 @@ struct LambdaAcc {
     }
 };
 
+// Neighbour loops fetch in groups of kGroup: all list entries of the group, then all gathered
+// operands, then the group is processed in list order (the summation order of the reference's
+// loop).  Entries past cnt re-read the last valid entry and are skipped, so every fetch is in
+// bounds and branch-free.  kGroup independent loads in flight per lane instead of one.
+constexpr int kGroup = 8;
+
+// Calls f(j, q, x_q) for j = 0..cnt-1 in order, q = L[j * stride], x_q = pos[q].
+template <class F>
+__device__ __forceinline__ void for_neighbors(const uint32_t* __restrict__ L, int stride, int cnt,
+                                              const float4* __restrict__ pos, F&& f) {
+    for (int j0 = 0; j0 < cnt; j0 += kGroup) {
+        uint32_t q[kGroup];
+#pragma unroll
+        for (int u = 0; u < kGroup; ++u) q[u] = L[(size_t)min(j0 + u, cnt - 1) * stride];
+        float4 x[kGroup];
+#pragma unroll
+        for (int u = 0; u < kGroup; ++u) x[u] = pos[q[u]];
+#pragma unroll
+        for (int u = 0; u < kGroup; ++u)
+            if (j0 + u < cnt) f(q[u], x[u]);
+    }
+}
+
 // Block sum of per-thread values in a fixed tree order into partial[blockIdx.x].
 template <int BS>
 __device__ void block_sum(float c, float* __restrict__ partial) {
@@ -313,15 +336,22 @@ __global__ __launch_bounds__(BS) void k_neighbors_lambda(StepConsts K, const Sce
                 if (cy < 0 || cy >= G.dims[1] || xlo > xhi) continue;
                 int row = G.coff + (cz * G.dims[1] + cy) * G.dims[0];
                 uint32_t b = cstart[row + xlo], e = cstart[row + xhi + 1];
-                for (uint32_t q = b; q < e && cnt < K.cap; ++q) {
-                    if ((int)q == s) continue;
-                    float4 xj = pos[q];
-                    float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-                    float r2 = rx * rx + ry * ry + rz * rz;
-                    if (r2 <= K.h2) {
-                        out[(size_t)cnt * K.nstride] = q;
-                        ++cnt;
-                        A.add(rx, ry, rz, r2, K);
+                for (uint32_t q0 = b; q0 < e && cnt < K.cap; q0 += kGroup) {
+                    float4 xs[kGroup];
+#pragma unroll
+                    for (int u = 0; u < kGroup; ++u) xs[u] = pos[min(q0 + (uint32_t)u, e - 1u)];
+#pragma unroll
+                    for (int u = 0; u < kGroup; ++u) {
+                        const uint32_t q = q0 + (uint32_t)u;
+                        if (q >= e || cnt >= K.cap || (int)q == s) continue;
+                        const float4 xj = xs[u];
+                        float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+                        float r2 = rx * rx + ry * ry + rz * rz;
+                        if (r2 <= K.h2) {
+                            out[(size_t)cnt * K.nstride] = q;
+                            ++cnt;
+                            A.add(rx, ry, rz, r2, K);
+                        }
                     }
                 }
             }
@@ -378,13 +408,11 @@ __global__ __launch_bounds__(BS) void k_lambda(StepConsts K, float4* __restrict_
         int cnt = ncnt[s];
         LambdaAcc A;
         A.init();
-        const uint32_t* Lst = nbr + s;
-        for (int j = 0; j < cnt; ++j) {
-            float4 xj = pos[Lst[(size_t)j * K.nstride]];
+        for_neighbors(nbr + s, K.nstride, cnt, pos, [&](uint32_t, const float4& xj) {
             float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
             float r2 = rx * rx + ry * ry + rz * rz;
             A.add(rx, ry, rz, r2, K);
-        }
+        });
         reinterpret_cast<float*>(pos + s)[3] = A.lambda(K);
         comp = comp + fmaxf(A.rho / K.restRho - 1.0f, 0.0f);
     }
@@ -409,9 +437,7 @@ __global__ __launch_bounds__(BS) void k_delta_apply(StepConsts K, const SceneGeo
         int cnt = ncnt[s];
         if (LA.lam_sum) LA.lam_sum[s] = (first ? 0.0f : LA.lam_sum[s]) + xi.w;
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
-        const uint32_t* Lst = nbr + s;
-        for (int j = 0; j < cnt; ++j) {
-            float4 xj = pos[Lst[(size_t)j * K.nstride]];
+        for_neighbors(nbr + s, K.nstride, cnt, pos, [&](uint32_t, const float4& xj) {
             float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
             float r2 = rx * rx + ry * ry + rz * rz;
             float w = poly6(r2, K);
@@ -420,7 +446,7 @@ __global__ __launch_bounds__(BS) void k_delta_apply(StepConsts K, const SceneGeo
             ax = ax + a * (rx * t);
             ay = ay + a * (ry * t);
             az = az + a * (rz * t);
-        }
+        });
         float x = xi.x + ax / K.restRho;
         float y = xi.y + ay / K.restRho;
         float z = xi.z + az / K.restRho;
@@ -451,19 +477,30 @@ __global__ __launch_bounds__(BS) void k_curl(StepConsts K, const float4* __restr
         int cnt = ncnt[s];
         float wx = 0.0f, wy = 0.0f, wz = 0.0f, rho = 0.0f;
         const uint32_t* Lst = nbr + s;
-        for (int j = 0; j < cnt; ++j) {
-            uint32_t q = Lst[(size_t)j * K.nstride];
-            float4 xj = pos[q];
-            float4 vj = vel[q];
-            float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
-            float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-            float r2 = rx * rx + ry * ry + rz * rz;
-            float t = spiky_scale(r2, K);
-            float gx = rx * t, gy = ry * t, gz = rz * t;
-            wx = wx + (uy * gz - uz * gy);
-            wy = wy + (uz * gx - ux * gz);
-            wz = wz + (ux * gy - uy * gx);
-            rho = rho + poly6(r2, K);
+        for (int j0 = 0; j0 < cnt; j0 += kGroup) {
+            uint32_t qs[kGroup];
+            float4 xs[kGroup], vs[kGroup];
+#pragma unroll
+            for (int u = 0; u < kGroup; ++u) qs[u] = Lst[(size_t)min(j0 + u, cnt - 1) * K.nstride];
+#pragma unroll
+            for (int u = 0; u < kGroup; ++u) {
+                xs[u] = pos[qs[u]];
+                vs[u] = vel[qs[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < kGroup; ++u) {
+                if (j0 + u >= cnt) continue;
+                const float4 xj = xs[u], vj = vs[u];
+                float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
+                float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+                float r2 = rx * rx + ry * ry + rz * rz;
+                float t = spiky_scale(r2, K);
+                float gx = rx * t, gy = ry * t, gz = rz * t;
+                wx = wx + (uy * gz - uz * gy);
+                wy = wy + (uz * gx - ux * gz);
+                wz = wz + (ux * gy - uy * gx);
+                rho = rho + poly6(r2, K);
+            }
         }
         omega[s] = make_float4(wx, wy, wz, sqrtf(wx * wx + wy * wy + wz * wz));
         float e = rho / K.restRho - 1.0f;
@@ -512,22 +549,36 @@ __global__ __launch_bounds__(BS) void k_vort_apply(StepConsts K, const float4* _
         int cnt = ncnt[s];
         float px = 0.0f, py = 0.0f, pz = 0.0f, ex = 0.0f, ey = 0.0f, ez = 0.0f;
         const uint32_t* Lst = nbr + s;
-        for (int j = 0; j < cnt; ++j) {
-            uint32_t q = Lst[(size_t)j * K.nstride];
-            float4 xj = pos[q];
-            float4 vj = vel[q];
-            float wl = omega[q].w;
-            float ux = vi.x - vj.x, uy = vi.y - vj.y, uz = vi.z - vj.z;
-            float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-            float r2 = rx * rx + ry * ry + rz * rz;
-            float w = poly6(r2, K);
-            px = px - ux * w;
-            py = py - uy * w;
-            pz = pz - uz * w;
-            float t = spiky_scale(r2, K);
-            ex = ex + wl * (rx * t);
-            ey = ey + wl * (ry * t);
-            ez = ez + wl * (rz * t);
+        const float* wlen = reinterpret_cast<const float*>(omega);
+        for (int j0 = 0; j0 < cnt; j0 += kGroup) {
+            uint32_t qs[kGroup];
+            float4 xs[kGroup], vs[kGroup];
+            float ws[kGroup];
+#pragma unroll
+            for (int u = 0; u < kGroup; ++u) qs[u] = Lst[(size_t)min(j0 + u, cnt - 1) * K.nstride];
+#pragma unroll
+            for (int u = 0; u < kGroup; ++u) {
+                xs[u] = pos[qs[u]];
+                vs[u] = vel[qs[u]];
+                ws[u] = wlen[4 * (size_t)qs[u] + 3];
+            }
+#pragma unroll
+            for (int u = 0; u < kGroup; ++u) {
+                if (j0 + u >= cnt) continue;
+                const float4 xj = xs[u], vj = vs[u];
+                const float wl = ws[u];
+                float ux = vi.x - vj.x, uy = vi.y - vj.y, uz = vi.z - vj.z;
+                float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+                float r2 = rx * rx + ry * ry + rz * rz;
+                float w = poly6(r2, K);
+                px = px - ux * w;
+                py = py - uy * w;
+                pz = pz - uz * w;
+                float t = spiky_scale(r2, K);
+                ex = ex + wl * (rx * t);
+                ey = ey + wl * (ry * t);
+                ez = ez + wl * (rz * t);
+            }
         }
         float el = sqrtf(ex * ex + ey * ey + ez * ez);
         if ((double)fabsf(el) > 1e-3) {
