@@ -26,7 +26,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from pbf_amd import FLAG_GRAPH, Solver, scenes  # noqa: E402
+from pbf_amd import FLAG_GRAPH, DeviceBuffer, Solver, lib, scenes  # noqa: E402
 
 METRIC = "ms/step + particle-steps/s @262k; time to 0.5% density err, learned vs plain"
 HBM_PEAK_GBS = 8000.0
@@ -64,6 +64,26 @@ def step_bytes(n: int, ncells: int, iters: int) -> float:
     return float(n * (252 + 44 * iters) + 12 * ncells)
 
 
+def copy_bandwidth(mb: int = 512, reps: int = 5) -> float:
+    """GB/s (read + write) of a plain streaming copy kernel: the counter-calibration reference and
+    the achievable-bandwidth context for the roofline."""
+    import ctypes as C
+
+    nbytes = mb << 20
+    a = DeviceBuffer(nbytes)
+    b = DeviceBuffer(nbytes)
+    best = None
+    ms = np.zeros(1, np.float32)
+    for _ in range(reps):
+        rc = lib().pbf_copy_kernel(b.ptr, a.ptr, nbytes, ms.ctypes.data_as(C.POINTER(C.c_float)))
+        if rc != 0:
+            raise RuntimeError(lib().pbf_last_error().decode())
+        best = ms[0] if best is None else min(best, ms[0])
+    a.close()
+    b.close()
+    return float(2.0 * nbytes / (float(best) * 1e-3) / 1e9)
+
+
 def cpu_baseline(scene, budget_s: float = 12.0):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Oracle, orc  # noqa: E402
@@ -99,6 +119,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--copy-mb", type=int, default=512, help="size of the calibration copy kernel (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -108,6 +129,7 @@ def main():
         )
         sys.exit(2)
 
+    copy_gbs = copy_bandwidth(args.copy_mb) if args.copy_mb > 0 else None
     scene = WORKLOADS[args.workload]()
     flags = 0 if args.no_graph else FLAG_GRAPH
     s = Solver.from_scene(scene, flags=flags)
@@ -183,6 +205,7 @@ def main():
         "stages_ms": {k: round(v, 5) for k, v in per_step_ms.items()},
         "step_roofline_frac": step_bytes(n, ncells, iters) / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9),
         "sort_reorder": {"ms": sort_ms, "frac": sort_bytes / (sort_ms * 1e-3) / (HBM_PEAK_GBS * 1e9) if sort_ms else None},
+        "copy_kernel_gbs": copy_gbs,
         "state_finite": finite,
         "cpu_baseline": cpu,
     }

@@ -225,22 +225,26 @@ int pbf_record_step(pbf_solver* s, int32_t iterations, float* dev_x, float* dev_
  * max(C_i, 0), C_i = rho_i/restRho - 1 over the frozen lists) is <= tol after at least
  * min_iters iterations, or after max_iters.  use_mlp: iteration 1 starts from the attached
  * network's multiplier instead of lambda.  The guess is kept if its residual is <= tol or below
- * the predicted state's, and plain iterations continue from it (the fallback that judges its
- * error); otherwise it is rejected and the step runs plain from the predicted state, exactly as
- * with use_mlp = 0.  Writes the iterations run (the guess counts as one) and the last residual
+ * the predicted state's and it does not scatter the fluid (mean |C| grows by at most 10%), and
+ * plain iterations continue from it (the fallback that judges its error); otherwise it is
+ * rejected and the step runs plain from the predicted state, exactly as with use_mlp = 0.  Writes the iterations run (the guess counts as one) and the last residual
  * measured (either pointer may be NULL).  Launched eagerly (one residual read per iteration). */
 int pbf_step_adaptive(pbf_solver* s, int32_t use_mlp, float tol, int32_t min_iters, int32_t max_iters,
                       int32_t* iters_used, float* resid);
 
-/* Of the last use_mlp step: out[0] = residual of the predicted state, out[1] = residual after the
- * guess, out[2] = 1 if the guess was kept, 0 if it was rejected. */
-int pbf_last_guess_info(pbf_solver* s, float out[3]);
+/* Of the last use_mlp step: out[0] / out[1] = compression residual of the predicted state /
+ * after the guess, out[2] = 1 if the guess was kept (0 if rejected), out[3] / out[4] = mean |C|
+ * of the predicted state / after the guess. */
+int pbf_last_guess_info(pbf_solver* s, float out[5]);
 
 /* Device memory helpers for callers without an allocator (datasets, export targets). */
 int pbf_device_alloc(uint64_t bytes, void** out);
 int pbf_device_free(void* p);
 int pbf_memcpy_to_host(void* dst, const void* src, uint64_t bytes);
 int pbf_memcpy_to_device(void* dst, const void* src, uint64_t bytes);
+/* Streaming copy kernel of `bytes` (multiple of 16) on the null stream, timed with HIP events:
+ * a known-traffic reference for hardware counter calibration. */
+int pbf_copy_kernel(void* dst, const void* src, uint64_t bytes, float* ms);
 
 /* Library build string. */
 const char* pbf_version(void);

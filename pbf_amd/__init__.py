@@ -172,6 +172,7 @@ def lib() -> C.CDLL:
         "pbf_device_free": (C.c_int, [_VP]),
         "pbf_memcpy_to_host": (C.c_int, [_VP, _VP, C.c_uint64]),
         "pbf_memcpy_to_device": (C.c_int, [_VP, _VP, C.c_uint64]),
+        "pbf_copy_kernel": (C.c_int, [_VP, _VP, C.c_uint64, _F]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -378,10 +379,11 @@ class Solver:
         return int(it[0]), float(r[0])
 
     def last_guess_info(self):
-        """(residual of the predicted state, residual after the guess, guess kept) of the last learned step."""
-        out = np.zeros(3, np.float32)
+        """(compression residual before / after the guess, guess kept, mean |C| before / after) of the
+        last learned step."""
+        out = np.zeros(5, np.float32)
         _check(lib().pbf_last_guess_info(self._h, _fp(out)))
-        return float(out[0]), float(out[1]), bool(out[2])
+        return float(out[0]), float(out[1]), bool(out[2]), float(out[3]), float(out[4])
 
     def record_step(self, iterations: int, ds: "DeviceDataset", base: int, stride: int, phase: int):
         _check(lib().pbf_record_step(self._h, int(iterations), ds.x, ds.t, int(base), int(ds.cap), int(stride), int(phase)))

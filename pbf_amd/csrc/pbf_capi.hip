@@ -78,7 +78,7 @@ struct pbf_solver {
     float* resid = nullptr;
     float* resid0 = nullptr;
     float* resid_host = nullptr;
-    float last_pred_resid = 0.0f, last_guess_resid = 0.0f;
+    float last_pred_resid = 0.0f, last_guess_resid = 0.0f, last_pred_dev = 0.0f, last_guess_dev = 0.0f;
     int guess_accepted = 0;
     const pbf::MlpModel* mlp = nullptr;
     // per-launch profiling: when set, enqueue_step records an event after every launch
@@ -345,9 +345,10 @@ int create_solver(const pbf_params* para, const float* grid_space, const float* 
         return fail(rc);
     s->npartial = (int)((np + 63) / 64) + 1;  // enough for the smallest workgroup size
     if ((rc = dalloc(s, &s->feat, np * kNumFeatures)) || (rc = dalloc(s, &s->lam_sum, np)) ||
-        (rc = dalloc(s, &s->resid, (size_t)s->npartial)) || (rc = dalloc(s, &s->resid0, (size_t)s->npartial)))
+        (rc = dalloc(s, &s->resid, (size_t)s->npartial * 2)) ||
+        (rc = dalloc(s, &s->resid0, (size_t)s->npartial * 2)))
         return fail(rc);
-    s->resid_host = (float*)std::malloc(sizeof(float) * s->npartial);
+    s->resid_host = (float*)std::malloc(sizeof(float) * s->npartial * 2);
     if (!s->resid_host) return fail(set_error(PBF_ERR_ARG, "host allocation failed"));
     if ((rc = dalloc(s, &s->dbg_gid, np)) || (rc = dalloc(s, &s->dbg_lam, np)) || (rc = dalloc(s, &s->dbg_err, np)) ||
         (rc = dalloc(s, &s->partial, (size_t)s->npartial * 3)))
@@ -865,14 +866,19 @@ uint64_t sm64(uint64_t& st) {
     return z ^ (z >> 31);
 }
 
-// Sum of the residual block partials / n (compression residual, mean max(C, 0)).
-int read_residual(pbf_solver* s, const float* partials, float* out) {
+// Means over particles of max(C, 0) (compression residual) and |C| from the workgroup partials.
+int read_residual(pbf_solver* s, const float* partials, float* comp, float* dev = nullptr) {
     int nb = reduce_blocks(s->tune, s->n);
-    HIP_TRY(hipMemcpyAsync(s->resid_host, partials, sizeof(float) * nb, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->resid_host, partials, sizeof(float) * 2 * nb, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
-    double acc = 0.0;
-    for (int b = 0; b < nb; ++b) acc += s->resid_host[b];
-    *out = (float)(acc / std::max(s->n, 1));
+    double a = 0.0, b = 0.0;
+    for (int k = 0; k < nb; ++k) {
+        a += s->resid_host[2 * k];
+        b += s->resid_host[2 * k + 1];
+    }
+    const double n = std::max(s->n, 1);
+    *comp = (float)(a / n);
+    if (dev) *dev = (float)(b / n);
     return PBF_OK;
 }
 
@@ -1089,6 +1095,24 @@ int pbf_device_alloc(uint64_t bytes, void** out) {
     return PBF_OK;
 }
 
+int pbf_copy_kernel(void* dst, const void* src, uint64_t bytes, float* ms) {
+    if (!dst || !src || (bytes % 16) != 0) return set_error(PBF_ERR_ARG, "bad argument (bytes must be a multiple of 16)");
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, nullptr));
+    launch_copy_u128(nullptr, src, dst, (size_t)bytes);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(e1, nullptr));
+    HIP_TRY(hipEventSynchronize(e1));
+    float t = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms) *ms = t;
+    return PBF_OK;
+}
+
 int pbf_device_free(void* p) {
     if (p) HIP_TRY(hipFree(p));
     return PBF_OK;
@@ -1167,13 +1191,18 @@ int pbf_step_adaptive(pbf_solver* s, int32_t use_mlp, float tol, int32_t min_ite
         launch_delta_apply(st, K, s->geo, s->tune, B.pos[a], B.pos[a ^ 1], B.vel[v], B, 0, 1, LA);
         launch_lambda(st, K, s->tune, B.pos[a ^ 1], B, LR);
         HIP_TRY(hipGetLastError());
-        float r0 = 0.0f, rg = 0.0f;
-        int rc = read_residual(s, s->resid0, &r0);
-        if (!rc) rc = read_residual(s, s->resid, &rg);
+        float r0 = 0.0f, rg = 0.0f, d0 = 0.0f, dg = 0.0f;
+        int rc = read_residual(s, s->resid0, &r0, &d0);
+        if (!rc) rc = read_residual(s, s->resid, &rg, &dg);
         if (rc) return rc;
         s->last_guess_resid = rg;
         s->last_pred_resid = r0;
-        if ((rg <= tol && min_iters <= 1) || rg < r0) {
+        s->last_guess_dev = dg;
+        s->last_pred_dev = d0;
+        // A guess is kept only if it does not scatter the fluid (mean |C| may not grow by more
+        // than 10%) and it reaches the tolerance or lowers the compression residual.
+        const bool sane = dg <= 1.1f * d0 + 1e-6f;
+        if (sane && ((rg <= tol && min_iters <= 1) || rg < r0)) {
             a ^= 1;
             it = 1;
             r = rg;
@@ -1212,13 +1241,16 @@ int pbf_step_adaptive(pbf_solver* s, int32_t use_mlp, float tol, int32_t min_ite
     return PBF_OK;
 }
 
-// Residuals of the last learned step: out[0] = predicted state, out[1] = after the guess,
-// out[2] = 1 if the guess was kept.
-int pbf_last_guess_info(pbf_solver* s, float out[3]) {
+// Residuals of the last learned step: out[0] / out[1] = compression residual of the predicted
+// state / after the guess, out[2] = 1 if the guess was kept, out[3] / out[4] = mean |C| of the
+// predicted state / after the guess.
+int pbf_last_guess_info(pbf_solver* s, float out[5]) {
     if (!s || !out) return set_error(PBF_ERR_ARG, "NULL argument");
     out[0] = s->last_pred_resid;
     out[1] = s->last_guess_resid;
     out[2] = (float)s->guess_accepted;
+    out[3] = s->last_pred_dev;
+    out[4] = s->last_guess_dev;
     return PBF_OK;
 }
 

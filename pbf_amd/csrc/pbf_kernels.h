@@ -40,7 +40,7 @@ struct LearnArgs {
     float* feat;         // [n][kNumFeatures] raw features (neighbour pass)
     const float4* vel;   // predicted velocity, multiplier of the previous step in .w (features)
     float* lam_sum;      // accumulated multiplier of this step (delta-p passes)
-    float* resid;        // block sums of max(C, 0) (lambda passes)
+    float* resid;        // per-workgroup (sum max(C, 0), sum |C|) pairs (lambda passes)
 };
 
 // Per-slot scene geometry of a batch (both NULL for a single scene).
@@ -85,5 +85,6 @@ void launch_export(hipStream_t st, int n, const float4* pos, const float4* vel, 
 void launch_import(hipStream_t st, int n, const float* pos_xyz, const float* vel_xyz, float4* pos, float4* vel,
                    uint32_t* ids);
 void launch_fill_u32(hipStream_t st, uint32_t* p, size_t m, uint32_t v);
+void launch_copy_u128(hipStream_t st, const void* src, void* dst, size_t bytes);
 
 }  // namespace pbf

@@ -23,6 +23,8 @@
 //
 // Reference kernels each stage replaces are cited at the kernel.
 
+#include <algorithm>
+
 #include "pbf_device.h"
 #include "pbf_kernels.h"
 
@@ -288,25 +290,32 @@ __device__ __forceinline__ void for_neighbors(const uint32_t* __restrict__ L, in
     }
 }
 
-// Block sum of per-thread values in a fixed tree order into partial[blockIdx.x].
+// Block sums of two per-thread values in a fixed tree order into partial[2 * blockIdx.x + {0,1}].
 template <int BS>
-__device__ void block_sum(float c, float* __restrict__ partial) {
-    __shared__ float sr[BS];
+__device__ void block_sum2(float c, float d, float* __restrict__ partial) {
+    __shared__ float sr[BS], sd[BS];
     int t = threadIdx.x;
     sr[t] = c;
+    sd[t] = d;
     __syncthreads();
     for (int w = BS / 2; w > 0; w >>= 1) {
-        if (t < w) sr[t] = sr[t] + sr[t + w];
+        if (t < w) {
+            sr[t] = sr[t] + sr[t + w];
+            sd[t] = sd[t] + sd[t + w];
+        }
         __syncthreads();
     }
-    if (t == 0) partial[blockIdx.x] = sr[0];
+    if (t == 0) {
+        partial[2 * blockIdx.x] = sr[0];
+        partial[2 * blockIdx.x + 1] = sd[0];
+    }
 }
 
 // buildNeighborhood (cu:124-154) fused with the first calculateLambda: the 27 cells in z, y, x
 // order (the three x-neighbours of a row are one contiguous slot run), self excluded,
 // r.r <= h*h, at most cap hits.  Lambda of iteration 1 goes to pos.w.  With L.feat the raw
 // warm-start features of the particle are written as well (kNumFeatures floats, see
-// pbf_kernels.h); with L.resid the workgroup sums of max(C, 0) at the predicted positions.
+// pbf_kernels.h); with L.resid the workgroup sums of max(C, 0) and |C| at the predicted positions.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_neighbors_lambda(StepConsts K, const SceneGeom* __restrict__ geoms,
                                                          const uint8_t* __restrict__ sid, float4* __restrict__ pos,
@@ -314,7 +323,7 @@ __global__ __launch_bounds__(BS) void k_neighbors_lambda(StepConsts K, const Sce
                                                          uint32_t* __restrict__ nbr, uint8_t* __restrict__ ncnt,
                                                          float* __restrict__ dbg_lam, LearnArgs L, int xcd_map) {
     const BlockRange R((K.n + BS - 1) / BS, xcd_map);
-    float comp = 0.0f;
+    float comp = 0.0f, dev = 0.0f;
     for (int blk = R.first; blk < R.last; ++blk) {
         const int s = blk * BS + threadIdx.x;
         if (s >= K.n) continue;
@@ -389,18 +398,20 @@ __global__ __launch_bounds__(BS) void k_neighbors_lambda(StepConsts K, const Sce
             o[2] = make_float4(f[8], f[9], f[10], f[11]);
             o[3] = make_float4(f[12], f[13], f[14], f[15]);
         }
-        comp = comp + fmaxf(A.rho / K.restRho - 1.0f, 0.0f);
+        const float C = A.rho / K.restRho - 1.0f;
+        comp = comp + fmaxf(C, 0.0f);
+        dev = dev + fabsf(C);
     }
-    if (L.resid) block_sum<BS>(comp, L.resid);
+    if (L.resid) block_sum2<BS>(comp, dev, L.resid);
 }
 
 // calculateLambda (cu:156-180) over the frozen list, iterations 2..I.  With L.resid, the
-// workgroup sums of max(C_i, 0) at the current positions are written as well.
+// workgroup sums of max(C_i, 0) and |C_i| at the current positions are written as well.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_lambda(StepConsts K, float4* __restrict__ pos, const uint32_t* __restrict__ nbr,
                                                const uint8_t* __restrict__ ncnt, LearnArgs L, int xcd_map) {
     const BlockRange R((K.n + BS - 1) / BS, xcd_map);
-    float comp = 0.0f;
+    float comp = 0.0f, dev = 0.0f;
     for (int blk = R.first; blk < R.last; ++blk) {
         const int s = blk * BS + threadIdx.x;
         if (s >= K.n) continue;
@@ -414,9 +425,11 @@ __global__ __launch_bounds__(BS) void k_lambda(StepConsts K, float4* __restrict_
             A.add(rx, ry, rz, r2, K);
         });
         reinterpret_cast<float*>(pos + s)[3] = A.lambda(K);
-        comp = comp + fmaxf(A.rho / K.restRho - 1.0f, 0.0f);
+        const float C = A.rho / K.restRho - 1.0f;
+        comp = comp + fmaxf(C, 0.0f);
+        dev = dev + fabsf(C);
     }
-    if (L.resid) block_sum<BS>(comp, L.resid);
+    if (L.resid) block_sum2<BS>(comp, dev, L.resid);
 }
 
 // calculateOffset (cu:182-199) + updatePosition (cu:15-22): delta-p with s_corr, then
@@ -637,6 +650,13 @@ __global__ __launch_bounds__(kBlock) void k_import(int n, const float* __restric
     ids[s] = (uint32_t)s;
 }
 
+// Plain streaming copy (counter calibration: reads and writes exactly `count` uint4).
+__global__ __launch_bounds__(kBlock) void k_copy_u128(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                      size_t count) {
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += (size_t)gridDim.x * kBlock)
+        dst[i] = src[i];
+}
+
 __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ p, size_t m, uint32_t v) {
     size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < m) p[i] = v;
@@ -736,6 +756,14 @@ void launch_export(hipStream_t st, int n, const float4* pos, const float4* vel, 
 void launch_import(hipStream_t st, int n, const float* pos_xyz, const float* vel_xyz, float4* pos, float4* vel,
                    uint32_t* ids) {
     if (n > 0) hipLaunchKernelGGL(k_import, dim3(nblk(n)), dim3(kBlock), 0, st, n, pos_xyz, vel_xyz, pos, vel, ids);
+}
+
+void launch_copy_u128(hipStream_t st, const void* src, void* dst, size_t bytes) {
+    size_t count = bytes / 16;
+    if (count == 0) return;
+    unsigned grid = (unsigned)std::min<size_t>((count + kBlock - 1) / kBlock, 8192);
+    hipLaunchKernelGGL(k_copy_u128, dim3(grid), dim3(kBlock), 0, st, reinterpret_cast<const uint4*>(src),
+                       reinterpret_cast<uint4*>(dst), count);
 }
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, size_t m, uint32_t v) {

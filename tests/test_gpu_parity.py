@@ -33,7 +33,7 @@ def pair(scene, flags=FLAG_DEBUG, **over):
     return Solver.from_scene(scene, flags=flags, **over), Oracle.from_scene(scene, **over)
 
 
-def compare_step(g, o, check_lists=True):
+def compare_step(g, o, check_lists=True, check_lambda=True):
     gg = g.grid()
     og = o.grid()
     nin = og[4]
@@ -50,6 +50,8 @@ def compare_step(g, o, check_lists=True):
     gf = g.fields()
     of = o.fields()
     for name, a, b in zip(("lambda first", "lambda last", "density error", "curl"), gf, of):
+        if name.startswith("lambda") and not check_lambda:
+            continue
         assert_bitexact(a, b, name)
     gp, gv = g.get_state()
     op, ov = o.get_state()
@@ -118,7 +120,8 @@ def test_iteration_counts(iters):
     g, o = pair(scenes.dam_break((16, 16, 16), (0, 0, 0, 1.6, 1.6, 1.6), seed=2), iterations=iters)
     g.step()
     o.step(1)
-    compare_step(g, o)
+    # with no iteration the reference computes no lambda (cudaPBF.cu:329)
+    compare_step(g, o, check_lambda=iters > 0)
 
 
 def test_reference_grid_64_cubed():
@@ -200,3 +203,31 @@ def test_set_get_state_roundtrip():
     p3, v3 = s.get_state()
     assert_bitexact(p3, p2, "positions round trip")
     assert_bitexact(v3, v2, "velocities round trip")
+
+
+def test_1000_steps_density_statistics_32k():
+    """1000-step trajectory: density-error statistics every 100 steps, and the final state."""
+    sc = scenes.config_cpu_32k()
+    g, o = pair(sc, flags=FLAG_DENSITY_STATS)
+    for k in range(10):
+        g.advance(100)
+        o.step(100)
+        st = g.density_stats()
+        err = o.fields()[2].astype(np.float64)
+        ref = (np.abs(err).mean(), np.maximum(err, 0).mean(), err.max())
+        assert st[2] == np.float32(ref[2]), f"max error at step {100 * (k + 1)}"
+        np.testing.assert_allclose(st[:2], ref[:2], rtol=1e-5, err_msg=f"step {100 * (k + 1)}")
+    gp, gv = g.get_state()
+    op, ov = o.get_state()
+    assert_bitexact(gp, op, "positions after 1000 steps")
+    assert_bitexact(gv, ov, "velocities after 1000 steps")
+
+
+def test_20_steps_262k_bitexact():
+    g, o = pair(scenes.config_262k(), flags=0)
+    g.advance(20)
+    o.step(20)
+    gp, gv = g.get_state()
+    op, ov = o.get_state()
+    assert_bitexact(gp, op, "positions")
+    assert_bitexact(gv, ov, "velocities")
