@@ -14,6 +14,7 @@
 //     truncated Taylor series), not libm's, so it is identical on both sides;
 //   * the spiky gradient is evaluated as r * ((cs*(h-|r|)^2)/|r|) -- the reference
 //     (cuh:105-110) evaluates (cs*(h-|r|)^2) * (r/|r|); the two agree to rounding;
+//   * s_corr's ratio W(r)/W(dq) (cuh:143) is evaluated as W(r) * (1/W(dq));
 //   * particles inside one cell are ordered by ascending particle index (the reference's
 //     atomicSub scatter, cu:112-122, leaves that order undefined).
 //
@@ -122,7 +123,7 @@ void bound_particle(V3& p, const float* bmin, const float* bmax, float coeff) {
 // Kernel constants, kernels/cudaPBFSolver.cuh:99-110: the reference evaluates
 // 315.0f*INVPI/(64.0f*powf(h,9)) and -45.0f*INVPI/powf(h,6) in double and rounds to float.
 struct Consts {
-    float h, h2, c6, cs, wdq, rmin, k, n, restRho, eps;
+    float h, h2, c6, cs, wdq, inv_wdq, rmin, k, n, restRho, eps;
     int n_int;  // -1 if n is not a small non-negative integer
 };
 
@@ -151,7 +152,7 @@ inline V3 spiky_grad(const V3& r, float r2, const Consts& K) {
 }
 
 inline float scorr(float w, const Consts& K) {
-    float ratio = w / K.wdq;
+    float ratio = w * K.inv_wdq;
     float p = K.n_int >= 0 ? powi(ratio, K.n_int) : std::pow(ratio, K.n);
     return -K.k * p;
 }
@@ -166,6 +167,7 @@ Consts make_consts(const orc_params& P) {
     K.cs = (float)((-45.0 * kInvPi) / (double)h6);
     float dq2 = P.deltaq * P.deltaq + 0.0f * 0.0f + 0.0f * 0.0f;
     K.wdq = poly6(dq2, K);
+    K.inv_wdq = 1.0f / K.wdq;
     K.rmin = P.grad_rmin;
     K.k = P.k;
     K.n = P.n;

@@ -108,6 +108,7 @@ StepConsts make_consts(const pbf_solver* s) {
         float d = K.h2 - dq2;
         K.wdq = K.c6 * (d * d * d);
     }
+    K.inv_wdq = 1.0f / K.wdq;
     K.k = P.k;
     K.scorr_n = P.n;
     K.n_int = (P.n >= 0.0f && P.n <= 64.0f && P.n == (float)(int)P.n) ? (int)P.n : -1;

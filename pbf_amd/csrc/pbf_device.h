@@ -24,7 +24,7 @@ struct SceneGeom {
 
 // Everything a step kernel needs, passed by value as one kernel argument.
 struct StepConsts {
-    float h, h2, c6, cs, wdq, k, scorr_n, restRho, eps, dt, visc_c, vort_eps, jitter, rmin;
+    float h, h2, c6, cs, wdq, inv_wdq, k, scorr_n, restRho, eps, dt, visc_c, vort_eps, jitter, rmin;
     int n_int;  // exponent of s_corr when it is a small non-negative integer, else -1
     float g[3];
     float gmin[3];
@@ -88,34 +88,37 @@ __device__ __forceinline__ void bound_particle(float& x, float& y, float& z, con
     if (z > G.bmax[2]) z = G.bmax[2] - K.jitter * random_offset(x, y);
 }
 
-// Wpoly6, cuh:99-103, on r.r.
+// Wpoly6, cuh:99-103, on r.r (evaluated, then selected: no branch).
 __device__ __forceinline__ float poly6(float r2, const StepConsts& K) {
-    if (r2 > K.h2) return 0.0f;
     float d = K.h2 - r2;
-    return K.c6 * (d * d * d);
+    float w = K.c6 * (d * d * d);
+    return r2 > K.h2 ? 0.0f : w;
 }
 
-// WspikyGrad, cuh:105-110, returned as the scalar t with grad = r * t.
+// WspikyGrad, cuh:105-110, returned as the scalar t with grad = r * t (evaluated, then
+// selected: no branch).
 __device__ __forceinline__ float spiky_scale(float r2, const StepConsts& K) {
     float len = sqrtf(r2);
-    if (len > K.h || len < K.rmin) return 0.0f;
     float hl = K.h - len;
-    return (K.cs * (hl * hl)) / len;
+    float t = (K.cs * (hl * hl)) / len;
+    return (len > K.h || len < K.rmin) ? 0.0f : t;
 }
 
+// b^e for 0 <= e < 128: the multiplications of square-and-multiply from the low bit up (the
+// exponent is uniform, so the branches are scalar).
 __device__ __forceinline__ float powi(float b, int e) {
     float r = 1.0f;
-    while (e) {
-        if (e & 1) r = r * b;
+#pragma unroll
+    for (int bit = 0; bit < 7; ++bit) {
+        if (e & (1 << bit)) r = r * b;
         b = b * b;
-        e >>= 1;
     }
     return r;
 }
 
-// Scorr, cuh:142-144, given W(r).
+// Scorr, cuh:142-144, given W(r): -k (W / W(dq))^n with the ratio as W * (1 / W(dq)).
 __device__ __forceinline__ float scorr(float w, const StepConsts& K) {
-    float ratio = w / K.wdq;
+    float ratio = w * K.inv_wdq;
     float p = K.n_int >= 0 ? powi(ratio, K.n_int) : powf(ratio, K.scorr_n);
     return -K.k * p;
 }
