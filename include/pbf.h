@@ -221,15 +221,15 @@ int pbf_attach_mlp(pbf_solver* s, pbf_mlp* m);
 int pbf_record_step(pbf_solver* s, int32_t iterations, float* dev_x, float* dev_t, int64_t base, int64_t cap,
                     int32_t stride, int32_t phase);
 
-/* One step whose constraint loop stops once the compression residual (mean over particles of
- * max(C_i, 0), C_i = rho_i/restRho - 1 over the frozen lists) is <= tol after at least
- * min_iters iterations, or after max_iters.  use_mlp: iteration 1 starts from the attached
+/* One step whose constraint loop stops once the residual is <= tol after at least min_iters
+ * iterations, or after max_iters.  metric 0: compression residual, the mean over particles of
+ * max(C_i, 0), C_i = rho_i/restRho - 1 over the frozen lists; metric 1: the mean of |C_i|.  use_mlp: iteration 1 starts from the attached
  * network's multiplier instead of lambda.  The guess is kept if its residual is <= tol or below
  * the predicted state's and it does not scatter the fluid (mean |C| grows by at most 10%), and
  * plain iterations continue from it (the fallback that judges its error); otherwise it is
  * rejected and the step runs plain from the predicted state, exactly as with use_mlp = 0.  Writes the iterations run (the guess counts as one) and the last residual
  * measured (either pointer may be NULL).  Launched eagerly (one residual read per iteration). */
-int pbf_step_adaptive(pbf_solver* s, int32_t use_mlp, float tol, int32_t min_iters, int32_t max_iters,
+int pbf_step_adaptive(pbf_solver* s, int32_t use_mlp, int32_t metric, float tol, int32_t min_iters, int32_t max_iters,
                       int32_t* iters_used, float* resid);
 
 /* Of the last use_mlp step: out[0] / out[1] = compression residual of the predicted state /

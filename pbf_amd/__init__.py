@@ -162,7 +162,7 @@ def lib() -> C.CDLL:
         "pbf_mlp_train": (C.c_int, [_VP, _VP, _VP, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_uint64, _F]),
         "pbf_attach_mlp": (C.c_int, [_VP, _VP]),
         "pbf_record_step": (C.c_int, [_VP, C.c_int32, _VP, _VP, C.c_int64, C.c_int64, C.c_int32, C.c_int32]),
-        "pbf_step_adaptive": (C.c_int, [_VP, C.c_int32, C.c_float, C.c_int32, C.c_int32, _I, _F]),
+        "pbf_step_adaptive": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_float, C.c_int32, C.c_int32, _I, _F]),
         "pbf_last_guess_info": (C.c_int, [_VP, _F]),
         "pbf_create_batch": (C.c_int, [C.POINTER(SolverParameter), C.c_int32, _I, _F, _F, _I, _F, _F, C.POINTER(_VP)]),
         "pbf_set_tuning": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32]),
@@ -371,11 +371,13 @@ class Solver:
         _check(lib().pbf_attach_mlp(self._h, None if mlp is None else mlp._h))
         self._mlp = mlp
 
-    def step_adaptive(self, use_mlp: bool, tol: float, min_iters: int, max_iters: int):
-        """One step stopping at compression residual <= tol; returns (iterations, residual)."""
+    def step_adaptive(self, use_mlp: bool, tol: float, min_iters: int, max_iters: int, metric: str = "compression"):
+        """One step stopping at residual <= tol (metric "compression": mean max(C,0); "abs":
+        mean |C|); returns (iterations, residual)."""
         it = np.zeros(1, np.int32)
         r = np.zeros(1, np.float32)
-        _check(lib().pbf_step_adaptive(self._h, int(bool(use_mlp)), float(tol), int(min_iters), int(max_iters), _ip(it), _fp(r)))
+        m = {"compression": 0, "abs": 1}[metric]
+        _check(lib().pbf_step_adaptive(self._h, int(bool(use_mlp)), m, float(tol), int(min_iters), int(max_iters), _ip(it), _fp(r)))
         return int(it[0]), float(r[0])
 
     def last_guess_info(self):

@@ -342,7 +342,7 @@ int create_solver(const pbf_params* para, const float* grid_space, const float* 
         (rc = dalloc(s, &B.omega, np)) || (rc = dalloc(s, &B.cell, np)) || (rc = dalloc(s, &B.local, np)) ||
         (rc = dalloc(s, &B.count, mpad)) || (rc = dalloc(s, &B.cstart, mpad + 4)) ||
         (rc = dalloc(s, &B.tile_sum, (size_t)scan_tiles(m) + 4)) || (rc = dalloc(s, &B.bucket, np)) ||
-        (rc = dalloc(s, &B.nbr, (size_t)s->cap * s->nstride)) || (rc = dalloc(s, &B.ncnt, np + 16)))
+        (rc = dalloc(s, &B.nbr, (size_t)((s->cap + 7) / 8) * 8 * s->nstride)) || (rc = dalloc(s, &B.ncnt, np + 16)))
         return fail(rc);
     s->npartial = (int)((np + 63) / 64) + 1;  // enough for the smallest workgroup size
     if ((rc = dalloc(s, &s->feat, np * kNumFeatures)) || (rc = dalloc(s, &s->lam_sum, np)) ||
@@ -594,7 +594,7 @@ int pbf_get_neighbors(pbf_solver* s, int32_t* nbr, int32_t* cnt) {
     if (s->n == 0) return PBF_OK;
     HIP_TRY(hipSetDevice(s->device));
     HIP_TRY(hipStreamSynchronize(s->stream));
-    std::vector<uint32_t> tab((size_t)s->cap * s->nstride), ids(s->n);
+    std::vector<uint32_t> tab((size_t)((s->cap + 7) / 8) * 8 * s->nstride), ids(s->n);
     std::vector<uint8_t> c(s->n);
     HIP_TRY(hipMemcpy(tab.data(), s->B.nbr, sizeof(uint32_t) * tab.size(), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(ids.data(), s->B.ids[s->ic], sizeof(uint32_t) * s->n, hipMemcpyDeviceToHost));
@@ -604,7 +604,8 @@ int pbf_get_neighbors(pbf_solver* s, int32_t* nbr, int32_t* cnt) {
         if (cnt) cnt[id] = c[sl];
         if (nbr)
             for (int j = 0; j < s->cap; ++j)
-                nbr[(size_t)id * s->cap + j] = j < c[sl] ? (int32_t)ids[tab[(size_t)j * s->nstride + sl]] : -1;
+                nbr[(size_t)id * s->cap + j] =
+                    j < c[sl] ? (int32_t)ids[tab[((size_t)(j / 8) * s->nstride + sl) * 8 + (j % 8)]] : -1;
     }
     return PBF_OK;
 }
@@ -1140,7 +1141,7 @@ int pbf_record_step(pbf_solver* s, int32_t iterations, float* dev_x, float* dev_
     HIP_TRY(hipSetDevice(s->device));
     float lam_min = 0.0f;
     int32_t used = 0;
-    int rc = pbf_step_adaptive(s, 0, lam_min, iterations, iterations, &used, nullptr);
+    int rc = pbf_step_adaptive(s, 0, 0, lam_min, iterations, iterations, &used, nullptr);
     if (rc) return rc;
     const float inv_h2 = 1.0f / (s->P.h * s->P.h);
     launch_record(s->stream, s->n, s->B.ids[s->ic], s->feat, s->lam_sum, inv_h2, stride, phase, (size_t)base,
@@ -1159,9 +1160,9 @@ int pbf_record_step(pbf_solver* s, int32_t iterations, float* dev_x, float* dev_
 // lambda is recomputed and the step runs plain from the predicted state, exactly as without the
 // network.  iters_used / resid (may be NULL) report the iterations run (the guess counts as one)
 // and the last residual measured.
-int pbf_step_adaptive(pbf_solver* s, int32_t use_mlp, float tol, int32_t min_iters, int32_t max_iters,
+int pbf_step_adaptive(pbf_solver* s, int32_t use_mlp, int32_t metric, float tol, int32_t min_iters, int32_t max_iters,
                       int32_t* iters_used, float* resid) {
-    if (!s || max_iters < 1 || min_iters < 0) return set_error(PBF_ERR_ARG, "bad argument");
+    if (!s || max_iters < 1 || min_iters < 0 || metric < 0 || metric > 1) return set_error(PBF_ERR_ARG, "bad argument");
     if (use_mlp && !s->mlp) return set_error(PBF_ERR_STATE, "no network attached");
     HIP_TRY(hipSetDevice(s->device));
     const StepConsts K = make_consts(s);
@@ -1200,15 +1201,16 @@ int pbf_step_adaptive(pbf_solver* s, int32_t use_mlp, float tol, int32_t min_ite
         s->last_pred_resid = r0;
         s->last_guess_dev = dg;
         s->last_pred_dev = d0;
+        const float m0 = metric ? d0 : r0, mg = metric ? dg : rg;
         // A guess is kept only if it does not scatter the fluid (mean |C| may not grow by more
-        // than 10%) and it reaches the tolerance or lowers the compression residual.
+        // than 10%) and it reaches the tolerance or lowers the residual.
         const bool sane = dg <= 1.1f * d0 + 1e-6f;
-        if (sane && ((rg <= tol && min_iters <= 1) || rg < r0)) {
+        if (sane && ((mg <= tol && min_iters <= 1) || mg < m0)) {
             a ^= 1;
             it = 1;
-            r = rg;
+            r = mg;
             s->guess_accepted = 1;
-            done = (rg <= tol && min_iters <= 1) || it >= max_iters;
+            done = (mg <= tol && min_iters <= 1) || it >= max_iters;
         } else {
             // rejected: restore lambda of iteration 1 at the predicted positions and run plain
             LearnArgs LN{};
@@ -1223,8 +1225,10 @@ int pbf_step_adaptive(pbf_solver* s, int32_t use_mlp, float tol, int32_t min_ite
         if (it >= max_iters) break;
         launch_lambda(st, K, s->tune, B.pos[a], B, LR);
         HIP_TRY(hipGetLastError());
-        int rc = read_residual(s, s->resid, &r);
+        float rc_comp = 0.0f, rc_dev = 0.0f;
+        int rc = read_residual(s, s->resid, &rc_comp, &rc_dev);
         if (rc) return rc;
+        r = metric ? rc_dev : rc_comp;
         if (it >= min_iters && r <= tol) break;
     }
     launch_velocity(st, K, B.pos[a], B.vel[v], B);

@@ -23,7 +23,7 @@ struct StepBuffers {
     uint32_t* cstart;    // ncells + 2
     uint32_t* tile_sum;  // scan tiles
     uint32_t* bucket;
-    uint32_t* nbr;       // cap * nstride
+    uint32_t* nbr;       // ceil(cap / 8) * 8 * nstride, blocks of 8 entries per slot
     uint8_t* ncnt;
 };
 
@@ -75,7 +75,7 @@ void launch_lambda(hipStream_t st, const StepConsts& K, const Tuning& T, float4*
                    const LearnArgs& L);
 void launch_delta_apply(hipStream_t st, const StepConsts& K, const Geometry& GE, const Tuning& T, const float4* pos,
                         float4* pos_next, float4* vel, const StepBuffers& B, int last, int first, const LearnArgs& L);
-void launch_curl(hipStream_t st, const StepConsts& K, const Tuning& T, const float4* pos, const float4* vel,
+void launch_curl(hipStream_t st, const StepConsts& K, const Tuning& T, const float4* pos, float4* vel,
                  const StepBuffers& B, float* err, float* partial);
 void launch_vort_apply(hipStream_t st, const StepConsts& K, const Tuning& T, const float4* pos, const float4* vel,
                        float4* vel_out, const StepBuffers& B, const float* lam_sum);

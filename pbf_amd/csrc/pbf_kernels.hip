@@ -7,8 +7,10 @@
 //                                     multiplier accumulated last step (warm start only)
 //   xold4  float4 (x_old)             position at the start of the step
 //   omega4 float4 (wx, wy, wz, |w|)   curl
-//   nbr    uint32 [cap][nstride]       neighbour slots, row j holds every particle's j-th
-//                                     neighbour (coalesced across a wavefront)
+//   nbr    uint32 blocks of 8          neighbour slots: entry j of slot s at
+//                                     ((j / 8) * nstride + s) * 8 + j % 8, so a particle's group
+//                                     of 8 is 32 contiguous bytes and a wavefront's group is one
+//                                     contiguous 2 KB range
 //   ncnt   uint8  [n]                  neighbour count
 //   count / cstart  uint32 [nbuckets] / [nbuckets+1]  cell histogram and its exclusive scan;
 //                                     each scene's last bucket collects its out-of-grid particles
@@ -267,20 +269,42 @@ struct LambdaAcc {
     }
 };
 
+__device__ __forceinline__ size_t nbr_index(int s, int j, int nstride) {
+    return ((size_t)(j >> 3) * (size_t)nstride + (size_t)s) * 8 + (size_t)(j & 7);
+}
+
+// The 8 list entries of group j0 (a multiple of 8) of slot s; entries past cnt are replaced by
+// s itself so every gather stays in bounds.
+__device__ __forceinline__ void load_group(const uint32_t* __restrict__ nbr, int s, int j0, int cnt, int nstride,
+                                           uint32_t q[8]) {
+    const uint4* p = reinterpret_cast<const uint4*>(nbr + nbr_index(s, j0, nstride));
+    const uint4 a = p[0], b = p[1];
+    q[0] = a.x;
+    q[1] = a.y;
+    q[2] = a.z;
+    q[3] = a.w;
+    q[4] = b.x;
+    q[5] = b.y;
+    q[6] = b.z;
+    q[7] = b.w;
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+        if (j0 + u >= cnt) q[u] = (uint32_t)s;
+}
+
 // Neighbour loops fetch in groups of kGroup: all list entries of the group, then all gathered
 // operands, then the group is processed in list order (the summation order of the reference's
 // loop).  Entries past cnt re-read the last valid entry and are skipped, so every fetch is in
 // bounds and branch-free.  kGroup independent loads in flight per lane instead of one.
 constexpr int kGroup = 8;
 
-// Calls f(j, q, x_q) for j = 0..cnt-1 in order, q = L[j * stride], x_q = pos[q].
+// Calls f(q, x_q) for the neighbours j = 0..cnt-1 of slot s in order, x_q = pos[q].
 template <class F>
-__device__ __forceinline__ void for_neighbors(const uint32_t* __restrict__ L, int stride, int cnt,
+__device__ __forceinline__ void for_neighbors(const uint32_t* __restrict__ nbr, int s, int nstride, int cnt,
                                               const float4* __restrict__ pos, F&& f) {
     for (int j0 = 0; j0 < cnt; j0 += kGroup) {
         uint32_t q[kGroup];
-#pragma unroll
-        for (int u = 0; u < kGroup; ++u) q[u] = L[(size_t)min(j0 + u, cnt - 1) * stride];
+        load_group(nbr, s, j0, cnt, nstride, q);
         float4 x[kGroup];
 #pragma unroll
         for (int u = 0; u < kGroup; ++u) x[u] = pos[q[u]];
@@ -336,7 +360,6 @@ __global__ __launch_bounds__(BS) void k_neighbors_lambda(StepConsts K, const Sce
         LambdaAcc A;
         A.init();
         int cnt = 0;
-        uint32_t* out = nbr + s;
         for (int dz = -1; dz <= 1; ++dz) {
             int cz = d2 + dz;
             if (cz < 0 || cz >= G.dims[2]) continue;
@@ -357,7 +380,7 @@ __global__ __launch_bounds__(BS) void k_neighbors_lambda(StepConsts K, const Sce
                         float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
                         float r2 = rx * rx + ry * ry + rz * rz;
                         if (r2 <= K.h2) {
-                            out[(size_t)cnt * K.nstride] = q;
+                            nbr[nbr_index(s, cnt, K.nstride)] = q;
                             ++cnt;
                             A.add(rx, ry, rz, r2, K);
                         }
@@ -419,7 +442,7 @@ __global__ __launch_bounds__(BS) void k_lambda(StepConsts K, float4* __restrict_
         int cnt = ncnt[s];
         LambdaAcc A;
         A.init();
-        for_neighbors(nbr + s, K.nstride, cnt, pos, [&](uint32_t, const float4& xj) {
+        for_neighbors(nbr, s, K.nstride, cnt, pos, [&](uint32_t, const float4& xj) {
             float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
             float r2 = rx * rx + ry * ry + rz * rz;
             A.add(rx, ry, rz, r2, K);
@@ -450,7 +473,7 @@ __global__ __launch_bounds__(BS) void k_delta_apply(StepConsts K, const SceneGeo
         int cnt = ncnt[s];
         if (LA.lam_sum) LA.lam_sum[s] = (first ? 0.0f : LA.lam_sum[s]) + xi.w;
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
-        for_neighbors(nbr + s, K.nstride, cnt, pos, [&](uint32_t, const float4& xj) {
+        for_neighbors(nbr, s, K.nstride, cnt, pos, [&](uint32_t, const float4& xj) {
             float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
             float r2 = rx * rx + ry * ry + rz * rz;
             float w = poly6(r2, K);
@@ -476,7 +499,7 @@ __global__ __launch_bounds__(BS) void k_delta_apply(StepConsts K, const SceneGeo
 // calculateCurl (cu:201-215), plus the density error rho/rho0 - 1 at the final positions when
 // err or partial is given (partial: per workgroup sum |e|, sum max(e,0), max e).
 template <int BS>
-__global__ __launch_bounds__(BS) void k_curl(StepConsts K, const float4* __restrict__ pos, const float4* __restrict__ vel,
+__global__ __launch_bounds__(BS) void k_curl(StepConsts K, const float4* __restrict__ pos, float4* __restrict__ vel,
                                              const uint32_t* __restrict__ nbr, const uint8_t* __restrict__ ncnt,
                                              float4* __restrict__ omega, float* __restrict__ err,
                                              float* __restrict__ partial, int xcd_map) {
@@ -489,12 +512,10 @@ __global__ __launch_bounds__(BS) void k_curl(StepConsts K, const float4* __restr
         const float4 vi = vel[s];
         int cnt = ncnt[s];
         float wx = 0.0f, wy = 0.0f, wz = 0.0f, rho = 0.0f;
-        const uint32_t* Lst = nbr + s;
         for (int j0 = 0; j0 < cnt; j0 += kGroup) {
             uint32_t qs[kGroup];
             float4 xs[kGroup], vs[kGroup];
-#pragma unroll
-            for (int u = 0; u < kGroup; ++u) qs[u] = Lst[(size_t)min(j0 + u, cnt - 1) * K.nstride];
+            load_group(nbr, s, j0, cnt, K.nstride, qs);
 #pragma unroll
             for (int u = 0; u < kGroup; ++u) {
                 xs[u] = pos[qs[u]];
@@ -515,7 +536,11 @@ __global__ __launch_bounds__(BS) void k_curl(StepConsts K, const float4* __restr
                 rho = rho + poly6(r2, K);
             }
         }
-        omega[s] = make_float4(wx, wy, wz, sqrtf(wx * wx + wy * wy + wz * wz));
+        const float wlen = sqrtf(wx * wx + wy * wy + wz * wz);
+        omega[s] = make_float4(wx, wy, wz, wlen);
+        // |omega| rides in vel.w for the vorticity pass (its gathers then need pos and vel only);
+        // this pass reads only .xyz of vel.
+        reinterpret_cast<float*>(vel + s)[3] = wlen;
         float e = rho / K.restRho - 1.0f;
         if (err) err[s] = e;
         sa = sa + fabsf(e);
@@ -561,25 +586,20 @@ __global__ __launch_bounds__(BS) void k_vort_apply(StepConsts K, const float4* _
         const float4 wi = omega[s];
         int cnt = ncnt[s];
         float px = 0.0f, py = 0.0f, pz = 0.0f, ex = 0.0f, ey = 0.0f, ez = 0.0f;
-        const uint32_t* Lst = nbr + s;
-        const float* wlen = reinterpret_cast<const float*>(omega);
         for (int j0 = 0; j0 < cnt; j0 += kGroup) {
             uint32_t qs[kGroup];
             float4 xs[kGroup], vs[kGroup];
-            float ws[kGroup];
-#pragma unroll
-            for (int u = 0; u < kGroup; ++u) qs[u] = Lst[(size_t)min(j0 + u, cnt - 1) * K.nstride];
+            load_group(nbr, s, j0, cnt, K.nstride, qs);
 #pragma unroll
             for (int u = 0; u < kGroup; ++u) {
                 xs[u] = pos[qs[u]];
                 vs[u] = vel[qs[u]];
-                ws[u] = wlen[4 * (size_t)qs[u] + 3];
             }
 #pragma unroll
             for (int u = 0; u < kGroup; ++u) {
                 if (j0 + u >= cnt) continue;
                 const float4 xj = xs[u], vj = vs[u];
-                const float wl = ws[u];
+                const float wl = vj.w;  // |omega_j|, left in vel.w by the curl pass
                 float ux = vi.x - vj.x, uy = vi.y - vj.y, uz = vi.z - vj.z;
                 float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
                 float r2 = rx * rx + ry * ry + rz * rz;
@@ -730,7 +750,7 @@ void launch_delta_apply(hipStream_t st, const StepConsts& K, const Geometry& GE,
                                           GE.sid, pos, pos_next, B.nbr, B.ncnt, B.xold, vel, last, first, L, T.xcd_map));
 }
 
-void launch_curl(hipStream_t st, const StepConsts& K, const Tuning& T, const float4* pos, const float4* vel,
+void launch_curl(hipStream_t st, const StepConsts& K, const Tuning& T, const float4* pos, float4* vel,
                  const StepBuffers& B, float* err, float* partial) {
     if (K.n <= 0) return;
     PBF_DISPATCH_BS(T, hipLaunchKernelGGL(k_curl<BS>, dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, vel, B.nbr, B.ncnt,
