@@ -10,8 +10,12 @@ solver's stream (hipGraph replay).  Prints ONE JSON line.
 
 roofline: the dominant kernel group of an eagerly launched step (HIP events on the solver's
 stream around it), with its algorithmic bytes per launch (fp32 compulsory traffic; neighbour
-gathers counted as cache hits).  cpu_baseline: the oracle's C++ restatement of the reference
-step on a bounded sample of the same workload, all host threads.
+gathers counted as cache hits); traffic: the HBM bytes per launch of that kernel from the
+committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary for this workload, when there is one.
+cpu_baseline: the oracle's C++ restatement of the reference step on a bounded sample of the same
+workload, all host threads.  learned: the warm start trained and evaluated on the GPU (record,
+train, error curves from identical states, graph-replayed step times), for the 262k dam break
+and for a 32^3 dam break in a 3.2 x 2.0 x 1.6 m box.
 """
 from __future__ import annotations
 
@@ -26,7 +30,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from pbf_amd import FLAG_GRAPH, DeviceBuffer, Solver, lib, scenes  # noqa: E402
+from pbf_amd import FLAG_GRAPH, DeviceBuffer, Solver, learned, lib, scenes  # noqa: E402
 
 METRIC = "ms/step + particle-steps/s @262k; time to 0.5% density err, learned vs plain"
 HBM_PEAK_GBS = 8000.0
@@ -62,6 +66,44 @@ def algorithmic_bytes(stage: str, n: int, ncells: int) -> float:
 def step_bytes(n: int, ncells: int, iters: int) -> float:
     """Whole-step algorithmic bytes: N*(252 + 44*I) + 12*C."""
     return float(n * (252 + 44 * iters) + 12 * ncells)
+
+
+# HIP kernel behind each timed stage group (traffic lookup)
+STAGE_KERNEL = {
+    "predict_hash": "k_predict_hash",
+    "neighbors_lambda": "k_neighbors_lambda",
+    "lambda": "k_lambda",
+    "delta_apply": "k_delta_apply",
+    "curl": "k_curl",
+    "vort_apply": "k_vort_apply",
+}
+
+
+def measured_traffic(workload: str, stage: str):
+    """HBM bytes per launch of the stage's kernel from profiles/traffic_<workload>.md (lines
+    'traffic_bytes_per_launch: <kernel> <bytes>'), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.md")
+    kern = STAGE_KERNEL.get(stage)
+    if kern is None or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch:" and parts[1] == kern:
+                return float(parts[2])
+    return None
+
+
+def learned_leg(n_side, box, train_seeds, eval_seeds, rec_steps, stride, warm):
+    ds, rows = learned.record_dataset(train_seeds, rec_steps, n_side, box, stride=stride, log=lambda *a: None)
+    mean, inv, ym, ys, nvalid = learned.normalise(ds, rows)
+    m, losses = learned.train(ds, nvalid, mean, inv, ym, ys, steps=3000, log=lambda *a: None)
+    ds.close()
+    _, summ = learned.evaluate(m, list(eval_seeds), warm, nstates=4, n_side=n_side, box=box, time_steps=100, log=lambda *a: None)
+    summ["samples"] = nvalid
+    summ["loss"] = [float(losses[:20].mean()), float(losses[-100:].mean())]
+    m.close()
+    return summ
 
 
 def copy_bandwidth(mb: int = 512, reps: int = 5) -> float:
@@ -120,6 +162,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--copy-mb", type=int, default=512, help="size of the calibration copy kernel (0: skip)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -165,6 +208,13 @@ def main():
     finite = bool(np.isfinite(pos).all() and np.isfinite(vel).all())
     s.close()
 
+    learned_res = None
+    if not args.no_learned and args.workload == "dam_break_262k":
+        learned_res = {
+            "dam_break_262k": learned_leg((64, 64, 64), (0, 0, 0, 6.4, 4.0, 3.2), range(100, 108), (200, 201), 300, 64, 60),
+            "dam_break_32k_box_3.2x2.0x1.6": learned_leg((32, 32, 32), (0, 0, 0, 3.2, 2.0, 1.6), range(100, 104), (200, 201), 150, 8, 40),
+        }
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene)
@@ -198,7 +248,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(args.workload, dom),
             "algorithmic_bytes_per_launch": dom_bytes,
             "avg_launch_ms": avg_ms[dom],
         },
@@ -208,6 +258,7 @@ def main():
         "copy_kernel_gbs": copy_gbs,
         "state_finite": finite,
         "cpu_baseline": cpu,
+        "learned": learned_res,
     }
     print(json.dumps(out), flush=True)
 

@@ -223,19 +223,31 @@ int pbf_record_step(pbf_solver* s, int32_t iterations, float* dev_x, float* dev_
 
 /* One step whose constraint loop stops once the residual is <= tol after at least min_iters
  * iterations, or after max_iters.  metric 0: compression residual, the mean over particles of
- * max(C_i, 0), C_i = rho_i/restRho - 1 over the frozen lists; metric 1: the mean of |C_i|.  use_mlp: iteration 1 starts from the attached
- * network's multiplier instead of lambda.  The guess is kept if its residual is <= tol or below
- * the predicted state's and it does not scatter the fluid (mean |C| grows by at most 10%), and
- * plain iterations continue from it (the fallback that judges its error); otherwise it is
- * rejected and the step runs plain from the predicted state, exactly as with use_mlp = 0.  Writes the iterations run (the guess counts as one) and the last residual
- * measured (either pointer may be NULL).  Launched eagerly (one residual read per iteration). */
+ * max(C_i, 0), C_i = rho_i/restRho - 1 over the frozen lists; metric 1: the mean of |C_i|.
+ * use_mlp: iteration 1 is the learned first pass: the attached network's multiplier replaces
+ * lambda for one delta-p pass, which is kept only if mean |C| does not grow (otherwise the pass
+ * reruns plain from the predicted state, exactly as with use_mlp = 0); plain iterations then
+ * continue until the tolerance holds (the fallback that judges the guess).  Writes the passes
+ * run (the learned pass counts as one) and the last residual measured (either pointer may be
+ * NULL).  Launched eagerly, one residual read-back per iteration. */
 int pbf_step_adaptive(pbf_solver* s, int32_t use_mlp, int32_t metric, float tol, int32_t min_iters, int32_t max_iters,
                       int32_t* iters_used, float* resid);
 
-/* Of the last use_mlp step: out[0] / out[1] = compression residual of the predicted state /
- * after the guess, out[2] = 1 if the guess was kept (0 if rejected), out[3] / out[4] = mean |C|
- * of the predicted state / after the guess. */
+/* Of the last learned first pass: out[0] / out[1] = compression of the predicted state / after
+ * the network's pass, out[2] = 1 if it was kept (0 if rejected), out[3] / out[4] = mean |C| of
+ * the predicted state / after the network's pass.  Synchronises the solver's stream. */
 int pbf_last_guess_info(pbf_solver* s, float out[5]);
+
+/* passes > 0 (a network attached): every following pbf_step / pbf_advance / pbf_time_steps runs
+ * the learned first pass followed by passes - 1 plain iterations, with the keep / reject decision
+ * made on the device (no read-back, graph-captured with PBF_FLAG_GRAPH).  passes = 0: plain
+ * steps with params.iterations iterations again. */
+int pbf_set_learned(pbf_solver* s, int32_t passes);
+
+/* save != 0: copy the current state (positions, velocities, particle order, and the per-particle
+ * warm-start value carried with the velocity) into one device-side snapshot of the solver;
+ * save == 0: restore the state from it.  Stream-ordered, no host copies. */
+int pbf_snapshot(pbf_solver* s, int32_t save);
 
 /* Device memory helpers for callers without an allocator (datasets, export targets). */
 int pbf_device_alloc(uint64_t bytes, void** out);

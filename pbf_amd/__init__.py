@@ -164,6 +164,8 @@ def lib() -> C.CDLL:
         "pbf_record_step": (C.c_int, [_VP, C.c_int32, _VP, _VP, C.c_int64, C.c_int64, C.c_int32, C.c_int32]),
         "pbf_step_adaptive": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_float, C.c_int32, C.c_int32, _I, _F]),
         "pbf_last_guess_info": (C.c_int, [_VP, _F]),
+        "pbf_set_learned": (C.c_int, [_VP, C.c_int32]),
+        "pbf_snapshot": (C.c_int, [_VP, C.c_int32]),
         "pbf_create_batch": (C.c_int, [C.POINTER(SolverParameter), C.c_int32, _I, _F, _F, _I, _F, _F, C.POINTER(_VP)]),
         "pbf_set_tuning": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32]),
         "pbf_get_tuning": (C.c_int, [_VP, _I]),
@@ -373,16 +375,30 @@ class Solver:
 
     def step_adaptive(self, use_mlp: bool, tol: float, min_iters: int, max_iters: int, metric: str = "compression"):
         """One step stopping at residual <= tol (metric "compression": mean max(C,0); "abs":
-        mean |C|); returns (iterations, residual)."""
+        mean |C|); tol < 0 runs exactly max_iters passes with no read-back.  Returns (passes,
+        last residual read, -1 if none)."""
         it = np.zeros(1, np.int32)
         r = np.zeros(1, np.float32)
         m = {"compression": 0, "abs": 1}[metric]
         _check(lib().pbf_step_adaptive(self._h, int(bool(use_mlp)), m, float(tol), int(min_iters), int(max_iters), _ip(it), _fp(r)))
         return int(it[0]), float(r[0])
 
+    def set_learned(self, passes: int):
+        """passes > 0: following steps run the learned first pass + passes - 1 plain iterations
+        (device-side keep / reject, graph-captured); 0: plain steps again."""
+        _check(lib().pbf_set_learned(self._h, int(passes)))
+
+    def snapshot(self):
+        """Save the state (with the per-particle warm-start value) in a device-side snapshot."""
+        _check(lib().pbf_snapshot(self._h, 1))
+
+    def restore(self):
+        """Restore the state saved by snapshot()."""
+        _check(lib().pbf_snapshot(self._h, 0))
+
     def last_guess_info(self):
-        """(compression residual before / after the guess, guess kept, mean |C| before / after) of the
-        last learned step."""
+        """(compression before / after the network's pass, pass kept, mean |C| before / after) of
+        the last learned first pass."""
         out = np.zeros(5, np.float32)
         _check(lib().pbf_last_guess_info(self._h, _fp(out)))
         return float(out[0]), float(out[1]), bool(out[2]), float(out[3]), float(out[4])

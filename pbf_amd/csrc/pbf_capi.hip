@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -78,14 +79,20 @@ struct pbf_solver {
     float* resid = nullptr;
     float* resid0 = nullptr;
     float* resid_host = nullptr;
-    float last_pred_resid = 0.0f, last_guess_resid = 0.0f, last_pred_dev = 0.0f, last_guess_dev = 0.0f;
-    int guess_accepted = 0;
+    int* gate = nullptr;          // device: 1 if the last learned first pass was kept
+    float* gate_stats = nullptr;  // device: compression / mean |C| before and after that pass
+    int learned_passes = 0;       // > 0: steps run the learned first pass + (passes - 1) plain
     const pbf::MlpModel* mlp = nullptr;
+    // one device-side copy of the state (pbf_snapshot)
+    float4* snap_pos = nullptr;
+    float4* snap_vel = nullptr;
+    uint32_t* snap_ids = nullptr;
+    bool snap_valid = false;
     // per-launch profiling: when set, enqueue_step records an event after every launch
     std::vector<hipEvent_t>* prof_events = nullptr;
     std::vector<const char*>* prof_names = nullptr;
-    // graph replay, one graph per state parity
-    hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
+    // graph replay, one graph per (step kind, state parity)
+    std::map<int, hipGraphExec_t> graphs;
     std::vector<void*> allocs;
 };
 
@@ -162,11 +169,8 @@ int dalloc(pbf_solver* s, T** p, size_t count) {
 void free_all(pbf_solver* s) {
     std::free(s->resid_host);
     s->resid_host = nullptr;
-    for (int i = 0; i < 4; ++i)
-        if (s->graph[i]) {
-            (void)hipGraphExecDestroy(s->graph[i]);
-            s->graph[i] = nullptr;
-        }
+    for (auto& g : s->graphs) (void)hipGraphExecDestroy(g.second);
+    s->graphs.clear();
     for (void* p : s->allocs) (void)hipFree(p);
     s->allocs.clear();
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -174,11 +178,8 @@ void free_all(pbf_solver* s) {
 }
 
 void drop_graphs(pbf_solver* s) {
-    for (int i = 0; i < 4; ++i)
-        if (s->graph[i]) {
-            (void)hipGraphExecDestroy(s->graph[i]);
-            s->graph[i] = nullptr;
-        }
+    for (auto& g : s->graphs) (void)hipGraphExecDestroy(g.second);
+    s->graphs.clear();
 }
 
 // Enqueue one step.  Stage events (7 stage boundaries + start) are recorded when ev != NULL.
@@ -198,12 +199,47 @@ int prof_mark(pbf_solver* s, const char* name) {
         if (rc_) return rc_;                \
     } while (0)
 
+// Learned first pass: the network's multiplier (from the features the neighbour pass wrote)
+// replaces iteration 1's lambda in pos[a].w and one delta-p pass runs on it.  The gate keeps the
+// result only if mean |C| did not grow; otherwise lambda 1 is restored and the pass reruns plain
+// (both fallback launches return at once when the guess was kept).  Leaves the positions after
+// the pass in pos[a ^ 1] and, with need_lambda, their lambda in .w and residual partials in resid.
+int enqueue_guess_pass(pbf_solver* s, const StepConsts& K, int a, int v, bool need_lambda) {
+    StepBuffers& B = s->B;
+    hipStream_t st = s->stream;
+    const LearnArgs L0{};
+    launch_mlp_infer(st, s->mlp, s->feat, s->n, B.pos[a], nullptr, s->P.h * s->P.h);
+    PROF("mlp_infer");
+    launch_delta_apply(st, K, s->geo, s->tune, B.pos[a], B.pos[a ^ 1], B.vel[v], B, 0, 1, L0);
+    PROF("delta_apply");
+    LearnArgs LR{};
+    LR.resid = s->resid;
+    launch_lambda(st, K, s->tune, B.pos[a ^ 1], B, LR);
+    PROF("lambda");
+    launch_guess_gate(st, s->resid0, s->resid, reduce_blocks(s->tune, s->n), s->n, s->gate, s->gate_stats);
+    launch_restore_lambda(st, s->gate, B.lam1, B.pos[a], s->n);
+    LearnArgs LS{};
+    LS.skip = s->gate;
+    launch_delta_apply(st, K, s->geo, s->tune, B.pos[a], B.pos[a ^ 1], B.vel[v], B, 0, 1, LS);
+    if (need_lambda) {
+        LS.resid = s->resid;
+        launch_lambda(st, K, s->tune, B.pos[a ^ 1], B, LS);
+    }
+    PROF("guess_gate_fallback");
+    HIP_TRY(hipGetLastError());
+    return PBF_OK;
+}
+
+// Enqueue one step: plain (P.iterations Jacobi iterations) or, with learned_passes > 0, the
+// learned first pass followed by learned_passes - 1 plain iterations.  Stage events (7 stage
+// boundaries + start) are recorded when ev != NULL.
 int enqueue_step(pbf_solver* s, hipEvent_t* ev) {
     const StepConsts K = make_consts(s);
     StepBuffers& B = s->B;
     hipStream_t st = s->stream;
     const bool dbg = (s->P.flags & PBF_FLAG_DEBUG) != 0;
     const bool stats = dbg || (s->P.flags & PBF_FLAG_DENSITY_STATS);
+    const bool learned = s->learned_passes > 0 && s->mlp;
     if (ev) HIP_TRY(hipEventRecord(ev[0], st));
     PROF("start");
     launch_predict_hash(st, K, s->geo, B.pos[s->pc], B.vel[s->vc], B);
@@ -219,12 +255,25 @@ int enqueue_step(pbf_solver* s, hipEvent_t* ev) {
     int a = s->pc ^ 1;
     int v = s->vc ^ 1;
     const LearnArgs L0{};
-    launch_neighbors_lambda(st, K, s->geo, s->tune, B.pos[a], B, dbg ? s->dbg_lam : nullptr, L0);
+    LearnArgs LF{};
+    if (learned) {
+        LF.feat = s->feat;
+        LF.vel = B.vel[v];
+        LF.resid = s->resid0;
+    }
+    launch_neighbors_lambda(st, K, s->geo, s->tune, B.pos[a], B, dbg ? s->dbg_lam : nullptr, LF);
     PROF("neighbors_lambda");
     if (ev) HIP_TRY(hipEventRecord(ev[4], st));
-    const int I = s->P.iterations;
-    for (int it = 0; it < I; ++it) {
-        if (it > 0) {
+    const int I = learned ? s->learned_passes : s->P.iterations;
+    int it = 0;
+    if (learned) {
+        int rc = enqueue_guess_pass(s, K, a, v, I > 1);
+        if (rc) return rc;
+        a ^= 1;
+        it = 1;
+    }
+    for (; it < I; ++it) {
+        if (it > (learned ? 1 : 0)) {
             launch_lambda(st, K, s->tune, B.pos[a], B, L0);
             PROF("lambda");
         }
@@ -232,11 +281,15 @@ int enqueue_step(pbf_solver* s, hipEvent_t* ev) {
         PROF("delta_apply");
         a ^= 1;
     }
+    if (learned && I == 1) {
+        launch_velocity(st, K, B.pos[a], B.vel[v], B);
+        PROF("velocity");
+    }
     if (ev) HIP_TRY(hipEventRecord(ev[5], st));
     launch_curl(st, K, s->tune, B.pos[a], B.vel[v], B, dbg ? s->dbg_err : nullptr, stats ? s->partial : nullptr);
     PROF("curl");
     if (ev) HIP_TRY(hipEventRecord(ev[6], st));
-    launch_vort_apply(st, K, s->tune, B.pos[a], B.vel[v], B.vel[v ^ 1], B, nullptr);
+    launch_vort_apply(st, K, s->tune, B.pos[a], B.vel[v], B.vel[v ^ 1], B);
     PROF("vort_apply");
     if (ev) HIP_TRY(hipEventRecord(ev[7], st));
     HIP_TRY(hipGetLastError());
@@ -253,9 +306,11 @@ int run_step(pbf_solver* s) {
     HIP_TRY(hipSetDevice(s->device));
     if (!(s->P.flags & PBF_FLAG_GRAPH) || (s->P.flags & PBF_FLAG_DEBUG)) return enqueue_step(s, nullptr);
     // A captured step bakes in which buffer of each pair it reads; vc is unchanged by a step,
-    // so (pc, ic) names the variant.
-    int key = s->pc * 2 + s->ic;
-    if (!s->graph[key]) {
+    // so (pc, ic) and the kind of step name the variant.
+    const bool learned = s->learned_passes > 0 && s->mlp;
+    const int key = ((learned ? s->learned_passes : 0) << 2) | (s->pc * 2 + s->ic);
+    auto found = s->graphs.find(key);
+    if (found == s->graphs.end()) {
         int pc0 = s->pc, vc0 = s->vc, ic0 = s->ic, ll0 = s->lam_last, sd0 = s->steps_done;
         HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
         int rc = enqueue_step(s, nullptr);
@@ -270,7 +325,7 @@ int run_step(pbf_solver* s) {
         e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) return set_error(PBF_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
-        s->graph[key] = ge;
+        found = s->graphs.emplace(key, ge).first;
         // capture only recorded the step; restore the parity and launch it for real below
         s->pc = pc0;
         s->vc = vc0;
@@ -278,11 +333,12 @@ int run_step(pbf_solver* s) {
         s->lam_last = ll0;
         s->steps_done = sd0;
     }
-    HIP_TRY(hipGraphLaunch(s->graph[key], s->stream));
+    HIP_TRY(hipGraphLaunch(found->second, s->stream));
     // advance the parity exactly as enqueue_step does
+    const int I = learned ? s->learned_passes : s->P.iterations;
     int a = s->pc ^ 1;
-    for (int it = 0; it < s->P.iterations; ++it) a ^= 1;
-    s->lam_last = s->P.iterations > 0 ? (a ^ 1) : -1;
+    for (int it = 0; it < I; ++it) a ^= 1;
+    s->lam_last = I > 0 ? (a ^ 1) : -1;
     s->pc = a;
     s->ic ^= 1;
     s->stats_valid = (s->P.flags & PBF_FLAG_DENSITY_STATS) != 0;
@@ -342,12 +398,14 @@ int create_solver(const pbf_params* para, const float* grid_space, const float* 
         (rc = dalloc(s, &B.omega, np)) || (rc = dalloc(s, &B.cell, np)) || (rc = dalloc(s, &B.local, np)) ||
         (rc = dalloc(s, &B.count, mpad)) || (rc = dalloc(s, &B.cstart, mpad + 4)) ||
         (rc = dalloc(s, &B.tile_sum, (size_t)scan_tiles(m) + 4)) || (rc = dalloc(s, &B.bucket, np)) ||
-        (rc = dalloc(s, &B.nbr, (size_t)((s->cap + 7) / 8) * 8 * s->nstride)) || (rc = dalloc(s, &B.ncnt, np + 16)))
+        (rc = dalloc(s, &B.nbr, (size_t)((s->cap + 7) / 8) * 8 * s->nstride)) || (rc = dalloc(s, &B.ncnt, np + 16)) ||
+        (rc = dalloc(s, &B.lam1, np)))
         return fail(rc);
     s->npartial = (int)((np + 63) / 64) + 1;  // enough for the smallest workgroup size
     if ((rc = dalloc(s, &s->feat, np * kNumFeatures)) || (rc = dalloc(s, &s->lam_sum, np)) ||
         (rc = dalloc(s, &s->resid, (size_t)s->npartial * 2)) ||
-        (rc = dalloc(s, &s->resid0, (size_t)s->npartial * 2)))
+        (rc = dalloc(s, &s->resid0, (size_t)s->npartial * 2)) || (rc = dalloc(s, &s->gate, 4)) ||
+        (rc = dalloc(s, &s->gate_stats, 4)))
         return fail(rc);
     s->resid_host = (float*)std::malloc(sizeof(float) * s->npartial * 2);
     if (!s->resid_host) return fail(set_error(PBF_ERR_ARG, "host allocation failed"));
@@ -1087,6 +1145,8 @@ int pbf_attach_mlp(pbf_solver* s, pbf_mlp* m) {
     if (m && m->device != s->device) return set_error(PBF_ERR_ARG, "network and solver are on different devices");
     if (m) HIP_TRY(hipStreamSynchronize(m->stream));
     s->mlp = m ? m->M : nullptr;
+    if (!m) s->learned_passes = 0;
+    drop_graphs(s);  // captured learned steps name the previous network
     return PBF_OK;
 }
 
@@ -1139,9 +1199,8 @@ int pbf_record_step(pbf_solver* s, int32_t iterations, float* dev_x, float* dev_
     if (!s || !dev_x || !dev_t || iterations < 1 || stride < 1 || phase < 0 || phase >= stride || base < 0)
         return set_error(PBF_ERR_ARG, "bad argument");
     HIP_TRY(hipSetDevice(s->device));
-    float lam_min = 0.0f;
     int32_t used = 0;
-    int rc = pbf_step_adaptive(s, 0, 0, lam_min, iterations, iterations, &used, nullptr);
+    int rc = pbf_step_adaptive(s, 0, 0, -1.0f, iterations, iterations, &used, nullptr);
     if (rc) return rc;
     const float inv_h2 = 1.0f / (s->P.h * s->P.h);
     launch_record(s->stream, s->n, s->B.ids[s->ic], s->feat, s->lam_sum, inv_h2, stride, phase, (size_t)base,
@@ -1150,16 +1209,16 @@ int pbf_record_step(pbf_solver* s, int32_t iterations, float* dev_x, float* dev_
     return PBF_OK;
 }
 
-// One step whose constraint loop stops as soon as the compression residual (mean of
-// max(C_i, 0) over the frozen lists at the current positions) is <= tol after at least
-// min_iters iterations, or after max_iters.
+// One step whose constraint loop stops as soon as the residual (metric 0: compression, the mean
+// of max(C_i, 0); metric 1: mean |C_i|; over the frozen lists at the current positions) is
+// <= tol after at least min_iters iterations, or after max_iters.  Each residual is read back to
+// the host, so this is the measuring variant; fixed-pass stepping is pbf_set_learned + pbf_step.
 //
-// With use_mlp (a network attached) iteration 1 starts from the network's multiplier.  The guess
-// is kept when its residual is <= tol, or below the residual of the predicted state (then plain
-// iterations continue from it, the fallback that judges its error).  Otherwise it is rejected:
-// lambda is recomputed and the step runs plain from the predicted state, exactly as without the
-// network.  iters_used / resid (may be NULL) report the iterations run (the guess counts as one)
-// and the last residual measured.
+// With use_mlp (a network attached) iteration 1 is the learned first pass: the network's
+// multiplier replaces lambda, and the pass is kept only if mean |C| does not grow (else it reruns
+// plain from the predicted state, exactly as without the network).  Plain iterations then continue
+// until the tolerance holds: they are the fallback that judges the guess.  iters_used / resid (may
+// be NULL) report the passes run (the learned pass counts as one) and the last residual measured.
 int pbf_step_adaptive(pbf_solver* s, int32_t use_mlp, int32_t metric, float tol, int32_t min_iters, int32_t max_iters,
                       int32_t* iters_used, float* resid) {
     if (!s || max_iters < 1 || min_iters < 0 || metric < 0 || metric > 1) return set_error(PBF_ERR_ARG, "bad argument");
@@ -1182,40 +1241,24 @@ int pbf_step_adaptive(pbf_solver* s, int32_t use_mlp, int32_t metric, float tol,
     LF.resid = use_mlp ? s->resid0 : nullptr;
     launch_neighbors_lambda(st, K, s->geo, s->tune, B.pos[a], B, dbg ? s->dbg_lam : nullptr, LF);
     LearnArgs LA{};
-    LA.lam_sum = s->lam_sum;
+    if (!use_mlp) LA.lam_sum = s->lam_sum;
     LearnArgs LR{};
     LR.resid = s->resid;
     int it = 0;
     float r = -1.0f;
     bool done = false;
     if (use_mlp) {
-        launch_mlp_infer(st, s->mlp, s->feat, s->n, B.pos[a], nullptr, s->P.h * s->P.h);
-        launch_delta_apply(st, K, s->geo, s->tune, B.pos[a], B.pos[a ^ 1], B.vel[v], B, 0, 1, LA);
-        launch_lambda(st, K, s->tune, B.pos[a ^ 1], B, LR);
-        HIP_TRY(hipGetLastError());
-        float r0 = 0.0f, rg = 0.0f, d0 = 0.0f, dg = 0.0f;
-        int rc = read_residual(s, s->resid0, &r0, &d0);
-        if (!rc) rc = read_residual(s, s->resid, &rg, &dg);
+        int rc = enqueue_guess_pass(s, K, a, v, max_iters > 1);
         if (rc) return rc;
-        s->last_guess_resid = rg;
-        s->last_pred_resid = r0;
-        s->last_guess_dev = dg;
-        s->last_pred_dev = d0;
-        const float m0 = metric ? d0 : r0, mg = metric ? dg : rg;
-        // A guess is kept only if it does not scatter the fluid (mean |C| may not grow by more
-        // than 10%) and it reaches the tolerance or lowers the residual.
-        const bool sane = dg <= 1.1f * d0 + 1e-6f;
-        if (sane && ((mg <= tol && min_iters <= 1) || mg < m0)) {
-            a ^= 1;
-            it = 1;
-            r = mg;
-            s->guess_accepted = 1;
-            done = (mg <= tol && min_iters <= 1) || it >= max_iters;
-        } else {
-            // rejected: restore lambda of iteration 1 at the predicted positions and run plain
-            LearnArgs LN{};
-            launch_neighbors_lambda(st, K, s->geo, s->tune, B.pos[a], B, nullptr, LN);
-            s->guess_accepted = 0;
+        a ^= 1;
+        it = 1;
+        done = it >= max_iters;
+        if (!done && tol >= 0.0f && it >= min_iters) {
+            float comp = 0.0f, dev = 0.0f;
+            rc = read_residual(s, s->resid, &comp, &dev);
+            if (rc) return rc;
+            r = metric ? dev : comp;
+            done = r <= tol;
         }
     }
     while (!done) {
@@ -1225,15 +1268,16 @@ int pbf_step_adaptive(pbf_solver* s, int32_t use_mlp, int32_t metric, float tol,
         if (it >= max_iters) break;
         launch_lambda(st, K, s->tune, B.pos[a], B, LR);
         HIP_TRY(hipGetLastError());
+        if (tol < 0.0f || it < min_iters) continue;  // nothing to test: no read-back
         float rc_comp = 0.0f, rc_dev = 0.0f;
         int rc = read_residual(s, s->resid, &rc_comp, &rc_dev);
         if (rc) return rc;
         r = metric ? rc_dev : rc_comp;
-        if (it >= min_iters && r <= tol) break;
+        if (r <= tol) break;
     }
     launch_velocity(st, K, B.pos[a], B.vel[v], B);
     launch_curl(st, K, s->tune, B.pos[a], B.vel[v], B, dbg ? s->dbg_err : nullptr, stats ? s->partial : nullptr);
-    launch_vort_apply(st, K, s->tune, B.pos[a], B.vel[v], B.vel[v ^ 1], B, s->lam_sum);
+    launch_vort_apply(st, K, s->tune, B.pos[a], B.vel[v], B.vel[v ^ 1], B);
     HIP_TRY(hipGetLastError());
     s->lam_last = a ^ 1;
     s->pc = a;
@@ -1246,16 +1290,61 @@ int pbf_step_adaptive(pbf_solver* s, int32_t use_mlp, int32_t metric, float tol,
     return PBF_OK;
 }
 
-// Residuals of the last learned step: out[0] / out[1] = compression residual of the predicted
-// state / after the guess, out[2] = 1 if the guess was kept, out[3] / out[4] = mean |C| of the
-// predicted state / after the guess.
+// The last learned first pass: out[0] / out[1] = compression of the predicted state / after the
+// network's pass, out[2] = 1 if that pass was kept, out[3] / out[4] = mean |C| before / after.
 int pbf_last_guess_info(pbf_solver* s, float out[5]) {
     if (!s || !out) return set_error(PBF_ERR_ARG, "NULL argument");
-    out[0] = s->last_pred_resid;
-    out[1] = s->last_guess_resid;
-    out[2] = (float)s->guess_accepted;
-    out[3] = s->last_pred_dev;
-    out[4] = s->last_guess_dev;
+    HIP_TRY(hipSetDevice(s->device));
+    float st4[4];
+    int g = 0;
+    HIP_TRY(hipMemcpyAsync(st4, s->gate_stats, sizeof(st4), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(&g, s->gate, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    out[0] = st4[0];
+    out[1] = st4[1];
+    out[2] = (float)g;
+    out[3] = st4[2];
+    out[4] = st4[3];
+    return PBF_OK;
+}
+
+// passes > 0: every following pbf_step / pbf_advance / pbf_time_steps runs the learned first pass
+// (network attached) and passes - 1 plain iterations; 0: back to P.iterations plain iterations.
+int pbf_set_learned(pbf_solver* s, int32_t passes) {
+    if (!s || passes < 0 || passes > 1000) return set_error(PBF_ERR_ARG, "bad argument");
+    if (passes > 0 && !s->mlp) return set_error(PBF_ERR_STATE, "no network attached");
+    s->learned_passes = passes;
+    return PBF_OK;
+}
+
+// save != 0: copy the state (positions, velocities with their .w, particle order) into the
+// solver's snapshot buffers; save == 0: restore it.  Both are stream-ordered device copies.
+int pbf_snapshot(pbf_solver* s, int32_t save) {
+    if (!s) return set_error(PBF_ERR_ARG, "solver is NULL");
+    HIP_TRY(hipSetDevice(s->device));
+    const size_t np = (size_t)std::max(s->n, 1);
+    if (!s->snap_pos) {
+        int rc;
+        if ((rc = dalloc(s, &s->snap_pos, np)) || (rc = dalloc(s, &s->snap_vel, np)) || (rc = dalloc(s, &s->snap_ids, np)))
+            return rc;
+    }
+    if (!save && !s->snap_valid) return set_error(PBF_ERR_STATE, "no snapshot saved");
+    float4* p = s->B.pos[s->pc];
+    float4* v = s->B.vel[s->vc];
+    uint32_t* id = s->B.ids[s->ic];
+    if (save) {
+        launch_copy_u128(s->stream, p, s->snap_pos, sizeof(float4) * np);
+        launch_copy_u128(s->stream, v, s->snap_vel, sizeof(float4) * np);
+        HIP_TRY(hipMemcpyAsync(s->snap_ids, id, sizeof(uint32_t) * np, hipMemcpyDeviceToDevice, s->stream));
+        s->snap_valid = true;
+    } else {
+        launch_copy_u128(s->stream, s->snap_pos, p, sizeof(float4) * np);
+        launch_copy_u128(s->stream, s->snap_vel, v, sizeof(float4) * np);
+        HIP_TRY(hipMemcpyAsync(id, s->snap_ids, sizeof(uint32_t) * np, hipMemcpyDeviceToDevice, s->stream));
+        s->lam_last = -1;
+        s->stats_valid = false;
+    }
+    HIP_TRY(hipGetLastError());
     return PBF_OK;
 }
 

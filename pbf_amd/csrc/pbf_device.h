@@ -123,6 +123,14 @@ __device__ __forceinline__ float scorr(float w, const StepConsts& K) {
     return -K.k * p;
 }
 
+// scorr for n = 4 (the configured exponent): powi's multiplications for e = 4 are
+// 1 * ((b * b) * (b * b)), and 1 * x == x, so this is the same value with the exponent fixed.
+__device__ __forceinline__ float scorr4(float w, const StepConsts& K) {
+    float ratio = w * K.inv_wdq;
+    float b2 = ratio * ratio;
+    return -K.k * (b2 * b2);
+}
+
 // computeDimID, cuh:156-160: truncation toward zero of (p - pMin) * invH, clamped to
 // [-3, dim+2] first (no in-range cell or 27-cell search changes), NaN to a far cell.
 __device__ __forceinline__ int dim_of(float p, float pmin, float invH, int dim) {

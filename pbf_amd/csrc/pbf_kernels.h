@@ -25,12 +25,13 @@ struct StepBuffers {
     uint32_t* bucket;
     uint32_t* nbr;       // ceil(cap / 8) * 8 * nstride, blocks of 8 entries per slot
     uint8_t* ncnt;
+    float* lam1;         // iteration-1 lambda per slot
 };
 
 // Warm-start features per particle (raw fp32, written by the neighbour pass when requested):
 //  0 C_i = rho/rho0 - 1 at the predicted position   1 neighbour count
 //  2 sum_j |grad_j C_i|^2                             3 |grad_i C_i|
-//  4-6 v * dt / h (predicted velocity)                7 previous step's multiplier / h^2
+//  4-6 v * dt / h (predicted velocity)                7 previous step's iteration-1 lambda / h^2
 //  8 distance to the nearest wall / h (<= 4)          9-14 distance to each wall (x-,x+,y-,y+,z-,z+) / h
 //  15 lambda of iteration 1 / h^2
 constexpr int kNumFeatures = 16;
@@ -38,9 +39,10 @@ constexpr int kNumFeatures = 16;
 // Optional outputs of the constraint kernels (all NULL on the plain path).
 struct LearnArgs {
     float* feat;         // [n][kNumFeatures] raw features (neighbour pass)
-    const float4* vel;   // predicted velocity, multiplier of the previous step in .w (features)
+    const float4* vel;   // predicted velocity, previous step's iteration-1 lambda in .w (features)
     float* lam_sum;      // accumulated multiplier of this step (delta-p passes)
     float* resid;        // per-workgroup (sum max(C, 0), sum |C|) pairs (lambda passes)
+    const int* skip;     // delta-p / lambda passes return at once when *skip != 0
 };
 
 // Per-slot scene geometry of a batch (both NULL for a single scene).
@@ -78,7 +80,10 @@ void launch_delta_apply(hipStream_t st, const StepConsts& K, const Geometry& GE,
 void launch_curl(hipStream_t st, const StepConsts& K, const Tuning& T, const float4* pos, float4* vel,
                  const StepBuffers& B, float* err, float* partial);
 void launch_vort_apply(hipStream_t st, const StepConsts& K, const Tuning& T, const float4* pos, const float4* vel,
-                       float4* vel_out, const StepBuffers& B, const float* lam_sum);
+                       float4* vel_out, const StepBuffers& B);
+// learned first pass: accept / reject on the device, and the fallback's lambda restore
+void launch_guess_gate(hipStream_t st, const float* r0, const float* rg, int np, int n, int* gate, float* stats);
+void launch_restore_lambda(hipStream_t st, const int* gate, const float* lam1, float4* pos, int n);
 void launch_velocity(hipStream_t st, const StepConsts& K, const float4* pos, float4* vel, const StepBuffers& B);
 void launch_export(hipStream_t st, int n, const float4* pos, const float4* vel, const uint32_t* ids, float* pos_xyz,
                    float* vel_xyz);

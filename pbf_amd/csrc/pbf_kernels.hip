@@ -94,7 +94,7 @@ __global__ __launch_bounds__(kBlock) void k_predict_hash(StepConsts K, const Sce
                                                          float4* __restrict__ vp, uint32_t* __restrict__ cell,
                                                          uint32_t* __restrict__ local, uint32_t* __restrict__ count) {
     int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= K.n) return;
+    if (s >= K.n) return;  // whole tail lanes leave; ballots below count the active lanes only
     const SceneGeom G = geom_of(K, geoms, sid, s);
     float4 x = pos[s];
     float4 v = vel[s];
@@ -106,7 +106,23 @@ __global__ __launch_bounds__(kBlock) void k_predict_hash(StepConsts K, const Sce
     x.z = x.z + K.dt * v.z;
     bound_particle(x.x, x.y, x.z, K, G);
     int c = cell_of(x.x, x.y, x.z, K, G);
-    uint32_t l = atomicAdd(&count[c], 1u);
+    // Histogram with one atomic per run of equal cells in the wavefront (slots are in the
+    // previous step's cell order, so runs are long): the run head adds the run length and every
+    // lane takes base + its rank in the run.  Any order stays correct; runs only save atomics.
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(c, 1, 64);
+    const bool head = lane == 0 || prev != c;
+    const uint64_t heads = __ballot(head);
+    const uint64_t active = __ballot(1);
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const int hpos = 63 - __clzll(heads & upto);                 // head of this lane's run
+    const uint64_t later = heads & ~upto;                          // heads after this lane
+    const int next = later ? (__ffsll((unsigned long long)later) - 1) : 64;
+    const int run_end = min(next, 64 - __clzll(active));           // one past the run's last lane
+    uint32_t base = 0;
+    if (head) base = atomicAdd(&count[c], (uint32_t)(run_end - lane));
+    base = __shfl(base, hpos, 64);
+    const uint32_t l = base + (uint32_t)(lane - hpos);
     xp[s] = make_float4(x.x, x.y, x.z, 0.0f);
     vp[s] = v;  // .w carries the previous step's accumulated multiplier (warm start)
     cell[s] = (uint32_t)c;
@@ -337,7 +353,8 @@ __device__ void block_sum2(float c, float d, float* __restrict__ partial) {
 
 // buildNeighborhood (cu:124-154) fused with the first calculateLambda: the 27 cells in z, y, x
 // order (the three x-neighbours of a row are one contiguous slot run), self excluded,
-// r.r <= h*h, at most cap hits.  Lambda of iteration 1 goes to pos.w.  With L.feat the raw
+// r.r <= h*h, at most cap hits.  Lambda of iteration 1 goes to pos.w and to lam1 (kept for the
+// learned step's fallback and, through vel.w, as the next step's feature).  With L.feat the raw
 // warm-start features of the particle are written as well (kNumFeatures floats, see
 // pbf_kernels.h); with L.resid the workgroup sums of max(C, 0) and |C| at the predicted positions.
 template <int BS>
@@ -345,7 +362,8 @@ __global__ __launch_bounds__(BS) void k_neighbors_lambda(StepConsts K, const Sce
                                                          const uint8_t* __restrict__ sid, float4* __restrict__ pos,
                                                          const uint32_t* __restrict__ cstart,
                                                          uint32_t* __restrict__ nbr, uint8_t* __restrict__ ncnt,
-                                                         float* __restrict__ dbg_lam, LearnArgs L, int xcd_map) {
+                                                         float* __restrict__ lam1, float* __restrict__ dbg_lam,
+                                                         LearnArgs L, int xcd_map) {
     const BlockRange R((K.n + BS - 1) / BS, xcd_map);
     float comp = 0.0f, dev = 0.0f;
     for (int blk = R.first; blk < R.last; ++blk) {
@@ -356,18 +374,49 @@ __global__ __launch_bounds__(BS) void k_neighbors_lambda(StepConsts K, const Sce
         int d0 = dim_of(xi.x, G.gmin[0], K.invH, G.dims[0]);
         int d1 = dim_of(xi.y, G.gmin[1], K.invH, G.dims[1]);
         int d2 = dim_of(xi.z, G.gmin[2], K.invH, G.dims[2]);
-        int xlo = max(d0 - 1, 0), xhi = min(d0 + 1, G.dims[0] - 1);
-        LambdaAcc A;
-        A.init();
+        const int xlo = max(d0 - 1, 0), xhi = min(d0 + 1, G.dims[0] - 1);
+        // Cell culling: a neighbour cell whose box lies farther than h from the particle cannot
+        // hold a neighbour, so it is left out of the run (the x-cells that remain are contiguous,
+        // and the list keeps the reference order).  Distances to the particle's own cell faces,
+        // less a margin of 1e-3 h that covers the rounding of the cell index, so that no cell
+        // holding a particle within h is ever culled.  Only for particles inside the grid.
+        const bool inside = d0 >= 0 && d0 < G.dims[0] && d1 >= 0 && d1 < G.dims[1] && d2 >= 0 && d2 < G.dims[2];
+        float fl[3] = {0.0f, 0.0f, 0.0f}, fh[3] = {0.0f, 0.0f, 0.0f};
+        if (inside) {
+            const float p3[3] = {xi.x, xi.y, xi.z};
+            const int d3[3] = {d0, d1, d2};
+            const float margin = 1e-3f * K.h;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float lo = G.gmin[a] + (float)d3[a] * K.h;
+                fl[a] = fmaxf(p3[a] - lo - margin, 0.0f);
+                fh[a] = fmaxf(lo + K.h - p3[a] - margin, 0.0f);
+            }
+        }
+        // Search: hits go to a register block of 8 list entries, written as 2 x uint4 when full.
+        // The lambda accumulation runs afterwards over the finished list (same order), so the
+        // search loop, whose candidate slots diverge across the wavefront, stays light.
         int cnt = 0;
+        uint32_t blk8[kGroup];
+#pragma unroll
+        for (int u = 0; u < kGroup; ++u) blk8[u] = (uint32_t)s;
         for (int dz = -1; dz <= 1; ++dz) {
             int cz = d2 + dz;
             if (cz < 0 || cz >= G.dims[2]) continue;
+            const float ez = dz < 0 ? fl[2] : (dz > 0 ? fh[2] : 0.0f);
             for (int dy = -1; dy <= 1; ++dy) {
                 int cy = d1 + dy;
                 if (cy < 0 || cy >= G.dims[1] || xlo > xhi) continue;
+                const float ey = dy < 0 ? fl[1] : (dy > 0 ? fh[1] : 0.0f);
+                const float eyz = ey * ey + ez * ez;
+                int rlo = xlo, rhi = xhi;
+                if (inside) {
+                    if (eyz > K.h2) continue;
+                    if (eyz + fl[0] * fl[0] > K.h2) rlo = d0;
+                    if (eyz + fh[0] * fh[0] > K.h2) rhi = d0;
+                }
                 int row = G.coff + (cz * G.dims[1] + cy) * G.dims[0];
-                uint32_t b = cstart[row + xlo], e = cstart[row + xhi + 1];
+                uint32_t b = cstart[row + rlo], e = cstart[row + rhi + 1];
                 for (uint32_t q0 = b; q0 < e && cnt < K.cap; q0 += kGroup) {
                     float4 xs[kGroup];
 #pragma unroll
@@ -380,17 +429,36 @@ __global__ __launch_bounds__(BS) void k_neighbors_lambda(StepConsts K, const Sce
                         float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
                         float r2 = rx * rx + ry * ry + rz * rz;
                         if (r2 <= K.h2) {
-                            nbr[nbr_index(s, cnt, K.nstride)] = q;
+                            const int k = cnt & (kGroup - 1);
+#pragma unroll
+                            for (int w = 0; w < kGroup; ++w) blk8[w] = (k == w) ? q : blk8[w];
                             ++cnt;
-                            A.add(rx, ry, rz, r2, K);
+                            if (k == kGroup - 1) {
+                                uint4* o = reinterpret_cast<uint4*>(nbr + nbr_index(s, cnt - kGroup, K.nstride));
+                                o[0] = make_uint4(blk8[0], blk8[1], blk8[2], blk8[3]);
+                                o[1] = make_uint4(blk8[4], blk8[5], blk8[6], blk8[7]);
+                            }
                         }
                     }
                 }
             }
         }
+        if (cnt & (kGroup - 1)) {  // the last, partial block (entries past cnt are never read)
+            uint4* o = reinterpret_cast<uint4*>(nbr + nbr_index(s, cnt & ~(kGroup - 1), K.nstride));
+            o[0] = make_uint4(blk8[0], blk8[1], blk8[2], blk8[3]);
+            o[1] = make_uint4(blk8[4], blk8[5], blk8[6], blk8[7]);
+        }
         ncnt[s] = (uint8_t)cnt;
+        LambdaAcc A;
+        A.init();
+        for_neighbors(nbr, s, K.nstride, cnt, pos, [&](uint32_t, const float4& xj) {
+            float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+            float r2 = rx * rx + ry * ry + rz * rz;
+            A.add(rx, ry, rz, r2, K);
+        });
         float lam = A.lambda(K);
         reinterpret_cast<float*>(pos + s)[3] = lam;
+        lam1[s] = lam;
         if (dbg_lam) dbg_lam[s] = lam;
         if (L.feat) {
             const float4 v = L.vel[s];
@@ -433,6 +501,7 @@ __global__ __launch_bounds__(BS) void k_neighbors_lambda(StepConsts K, const Sce
 template <int BS>
 __global__ __launch_bounds__(BS) void k_lambda(StepConsts K, float4* __restrict__ pos, const uint32_t* __restrict__ nbr,
                                                const uint8_t* __restrict__ ncnt, LearnArgs L, int xcd_map) {
+    if (L.skip && *L.skip) return;
     const BlockRange R((K.n + BS - 1) / BS, xcd_map);
     float comp = 0.0f, dev = 0.0f;
     for (int blk = R.first; blk < R.last; ++blk) {
@@ -458,13 +527,14 @@ __global__ __launch_bounds__(BS) void k_lambda(StepConsts K, float4* __restrict_
 // calculateOffset (cu:182-199) + updatePosition (cu:15-22): delta-p with s_corr, then
 // x + delta-p clamped into the next position buffer; the last iteration also writes
 // v = (x - x_old) / dt.  With LA.lam_sum the multiplier of this iteration is accumulated.
-template <int BS>
+template <int BS, int SN>
 __global__ __launch_bounds__(BS) void k_delta_apply(StepConsts K, const SceneGeom* __restrict__ geoms,
                                                     const uint8_t* __restrict__ sid, const float4* __restrict__ pos,
                                                     float4* __restrict__ pos_next, const uint32_t* __restrict__ nbr,
                                                     const uint8_t* __restrict__ ncnt, const float4* __restrict__ xold,
                                                     float4* __restrict__ vel, int last, int first, LearnArgs LA,
                                                     int xcd_map) {
+    if (LA.skip && *LA.skip) return;
     const BlockRange R((K.n + BS - 1) / BS, xcd_map);
     for (int blk = R.first; blk < R.last; ++blk) {
         const int s = blk * BS + threadIdx.x;
@@ -478,7 +548,7 @@ __global__ __launch_bounds__(BS) void k_delta_apply(StepConsts K, const SceneGeo
             float r2 = rx * rx + ry * ry + rz * rz;
             float w = poly6(r2, K);
             float t = spiky_scale(r2, K);
-            float a = (xi.w + xj.w) + scorr(w, K);
+            float a = (xi.w + xj.w) + (SN == 4 ? scorr4(w, K) : scorr(w, K));
             ax = ax + a * (rx * t);
             ay = ay + a * (ry * t);
             az = az + a * (rz * t);
@@ -570,12 +640,13 @@ __global__ __launch_bounds__(BS) void k_curl(StepConsts K, const float4* __restr
     }
 }
 
-// calculateVorticityAndViscosity (cu:217-239) + applyVorticityAndViscosity (cu:24-29).
+// calculateVorticityAndViscosity (cu:217-239) + applyVorticityAndViscosity (cu:24-29).  vel.w
+// leaves with the step's iteration-1 lambda (the next step's warm-start feature).
 template <int BS>
 __global__ __launch_bounds__(BS) void k_vort_apply(StepConsts K, const float4* __restrict__ pos,
                                                    const float4* __restrict__ vel, const float4* __restrict__ omega,
                                                    const uint32_t* __restrict__ nbr, const uint8_t* __restrict__ ncnt,
-                                                   float4* __restrict__ vel_out, const float* __restrict__ lam_sum,
+                                                   float4* __restrict__ vel_out, const float* __restrict__ lam1,
                                                    int xcd_map) {
     const BlockRange R((K.n + BS - 1) / BS, xcd_map);
     for (int blk = R.first; blk < R.last; ++blk) {
@@ -625,8 +696,56 @@ __global__ __launch_bounds__(BS) void k_vort_apply(StepConsts K, const float4* _
         float fx = K.vort_eps * cx, fy = K.vort_eps * cy, fz = K.vort_eps * cz;
         float vx = K.visc_c * px, vy = K.visc_c * py, vz = K.visc_c * pz;
         vel_out[s] = make_float4(vi.x + (K.dt * fx + vx), vi.y + (K.dt * fy + vy), vi.z + (K.dt * fz + vz),
-                                 lam_sum ? lam_sum[s] : 0.0f);
+                                 lam1[s]);
     }
+}
+
+// Acceptance of a learned first pass, on the device: reduces the per-workgroup (sum max(C,0),
+// sum |C|) pairs at the predicted positions (r0) and after the guess pass (rg) in a fixed order
+// and keeps the guess iff mean |C| did not grow.  gate[0] = 1 kept / 0 rejected; stats = the
+// four means (compression before, after, |C| before, after).
+__global__ __launch_bounds__(256) void k_guess_gate(const float* __restrict__ r0, const float* __restrict__ rg,
+                                                    int np, float inv_n, int* __restrict__ gate,
+                                                    float* __restrict__ stats) {
+    __shared__ float sh[4][256];
+    float a[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int i = threadIdx.x; i < np; i += 256) {
+        a[0] = a[0] + r0[2 * i];
+        a[1] = a[1] + rg[2 * i];
+        a[2] = a[2] + r0[2 * i + 1];
+        a[3] = a[3] + rg[2 * i + 1];
+    }
+    for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] = a[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] = sh[k][threadIdx.x] + sh[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const float c0 = sh[0][0] * inv_n, cg = sh[1][0] * inv_n, d0 = sh[2][0] * inv_n, dg = sh[3][0] * inv_n;
+        gate[0] = dg <= d0 ? 1 : 0;
+        stats[0] = c0;
+        stats[1] = cg;
+        stats[2] = d0;
+        stats[3] = dg;
+    }
+}
+
+// Rejected guess: put iteration 1's lambda back into pos.w (no-op when the guess was kept).
+__global__ __launch_bounds__(kBlock) void k_restore_lambda(const int* __restrict__ gate, const float* __restrict__ lam1,
+                                                           float4* __restrict__ pos, int n) {
+    if (*gate) return;
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s < n) reinterpret_cast<float*>(pos + s)[3] = lam1[s];
+}
+
+void launch_guess_gate(hipStream_t st, const float* r0, const float* rg, int np, int n, int* gate, float* stats) {
+    hipLaunchKernelGGL(k_guess_gate, dim3(1), dim3(256), 0, st, r0, rg, np, n > 0 ? 1.0f / (float)n : 0.0f, gate, stats);
+}
+
+void launch_restore_lambda(hipStream_t st, const int* gate, const float* lam1, float4* pos, int n) {
+    if (n > 0) hipLaunchKernelGGL(k_restore_lambda, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, gate, lam1, pos, n);
 }
 
 // updatePosition's velocity (cu:21) for a constraint loop that stopped early.
@@ -733,7 +852,7 @@ void launch_neighbors_lambda(hipStream_t st, const StepConsts& K, const Geometry
                              const StepBuffers& B, float* dbg_lam, const LearnArgs& L) {
     if (K.n <= 0) return;
     PBF_DISPATCH_BS(T, hipLaunchKernelGGL(k_neighbors_lambda<BS>, dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
-                                          GE.sid, pos, B.cstart, B.nbr, B.ncnt, dbg_lam, L, T.xcd_map));
+                                          GE.sid, pos, B.cstart, B.nbr, B.ncnt, B.lam1, dbg_lam, L, T.xcd_map));
 }
 
 void launch_lambda(hipStream_t st, const StepConsts& K, const Tuning& T, float4* pos, const StepBuffers& B,
@@ -746,8 +865,15 @@ void launch_lambda(hipStream_t st, const StepConsts& K, const Tuning& T, float4*
 void launch_delta_apply(hipStream_t st, const StepConsts& K, const Geometry& GE, const Tuning& T, const float4* pos,
                         float4* pos_next, float4* vel, const StepBuffers& B, int last, int first, const LearnArgs& L) {
     if (K.n <= 0) return;
-    PBF_DISPATCH_BS(T, hipLaunchKernelGGL(k_delta_apply<BS>, dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
-                                          GE.sid, pos, pos_next, B.nbr, B.ncnt, B.xold, vel, last, first, L, T.xcd_map));
+    if (K.n_int == 4) {
+        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_delta_apply<BS, 4>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K,
+                                              GE.geoms, GE.sid, pos, pos_next, B.nbr, B.ncnt, B.xold, vel, last, first,
+                                              L, T.xcd_map));
+    } else {
+        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_delta_apply<BS, 0>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K,
+                                              GE.geoms, GE.sid, pos, pos_next, B.nbr, B.ncnt, B.xold, vel, last, first,
+                                              L, T.xcd_map));
+    }
 }
 
 void launch_curl(hipStream_t st, const StepConsts& K, const Tuning& T, const float4* pos, float4* vel,
@@ -758,10 +884,10 @@ void launch_curl(hipStream_t st, const StepConsts& K, const Tuning& T, const flo
 }
 
 void launch_vort_apply(hipStream_t st, const StepConsts& K, const Tuning& T, const float4* pos, const float4* vel,
-                       float4* vel_out, const StepBuffers& B, const float* lam_sum) {
+                       float4* vel_out, const StepBuffers& B) {
     if (K.n <= 0) return;
     PBF_DISPATCH_BS(T, hipLaunchKernelGGL(k_vort_apply<BS>, dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, vel, B.omega,
-                                          B.nbr, B.ncnt, vel_out, lam_sum, T.xcd_map));
+                                          B.nbr, B.ncnt, vel_out, B.lam1, T.xcd_map));
 }
 
 void launch_velocity(hipStream_t st, const StepConsts& K, const float4* pos, float4* vel, const StepBuffers& B) {

@@ -2,15 +2,24 @@
 
     python -m pbf_amd.learned [--quick] [--out results.json]
 
-1. Record: for each training seed, run the dam break with plain steps of ``record_iters``
+1. Record: for each training seed, run the dam break with plain steps of ``record_iters`` (8)
    Jacobi iterations; every step records, for a rotating 1/stride subset of particles, the 16
-   raw features at the predicted positions and the multiplier the step accumulated over its
-   iterations (divided by h^2).  Everything stays in device memory.
+   raw features at the predicted positions (the neighbour pass's density, neighbour count,
+   gradient sums, velocity, the previous step's iteration-1 lambda, wall distances and this
+   step's iteration-1 lambda) and the multiplier the step accumulated over its 8 iterations,
+   divided by h^2.  Everything stays in device memory.
 2. Normalise: per-feature mean / inverse std and target mean / std from the recorded data.
 3. Train: Adam on random batches, BF16 MFMA forward / backward kernels, FP32 master weights.
-4. Evaluate on held-out seeds: after a common warm-up, step with a residual tolerance (mean
-   compression error max(C,0) <= tol) plain and learned; report iterations and wall time per
-   step.  The plain iterations remain the fallback of every learned step.
+   The network's output is scaled by a relaxation factor (``RELAX``): applied in one delta-p
+   pass, the full 8-iteration multiplier overshoots because the pass does not see how the
+   neighbours move, and 0.4-0.5 of it followed by one or two plain passes (which remove the
+   guess's particle-scale noise) gave the lowest error in a sweep on held-out scenes.
+4. Evaluate on held-out seeds, from identical states (device snapshots): the density error after
+   a plain step of I iterations and after a learned step of p passes (the network's pass, kept
+   only if mean |C| does not grow, then p - 1 plain iterations); the error target is what 8
+   plain iterations reach from the same state.  Wall time per step of plain-8 and of the learned
+   step with the fewest passes that reaches the target on every evaluated state is measured with
+   HIP events over graph-replayed steps.
 """
 from __future__ import annotations
 
@@ -20,15 +29,19 @@ import time
 
 import numpy as np
 
-from . import DeviceDataset, FLAG_DENSITY_STATS, Mlp, Solver, scenes
+from . import FLAG_DENSITY_STATS, FLAG_GRAPH, DeviceDataset, Mlp, Solver, scenes
+
+RELAX = 0.45
+TARGET_ITERS = 8
 
 
-def record_dataset(seeds, steps, n_side=(64, 64, 64), box=(0, 0, 0, 6.4, 4.0, 3.2), record_iters=8, stride=64, log=print):
+def record_dataset(seeds, steps, n_side=(64, 64, 64), box=(0, 0, 0, 6.4, 4.0, 3.2), record_iters=TARGET_ITERS,
+                   stride=64, log=print):
     per_step = None
     ds = None
     rows = 0
     t0 = time.perf_counter()
-    for si, seed in enumerate(seeds):
+    for seed in seeds:
         sc = scenes.dam_break(n_side, box, seed=seed)
         s = Solver.from_scene(sc)
         if per_step is None:
@@ -56,82 +69,130 @@ def normalise(ds: DeviceDataset, rows: int):
     return mean.astype(np.float32), inv, ym, ys, int(valid.sum())
 
 
-def train(ds, nsamples, mean, inv, ym, ys, steps=3000, batch=4096, lr=2e-3, seed=0, log=print):
+def train(ds, nsamples, mean, inv, ym, ys, steps=3000, batch=4096, lr=2e-3, seed=0, relax=RELAX, log=print):
+    """Train on the normalised dataset; the network's multiplier is relax * (y * ys + ym)."""
     m = Mlp(seed=seed)
-    m.set_normalization(mean, inv, ys, ym)
+    m.set_normalization(mean, inv, ys * relax, ym * relax)
     t0 = time.perf_counter()
     losses = m.train(ds, nsamples, batch=batch, steps=steps, lr=lr, lr_final=lr * 0.02, seed=seed + 1)
     log(f"  trained {steps} steps in {time.perf_counter() - t0:.2f} s: loss {losses[:20].mean():.4f} -> {losses[-100:].mean():.4f}")
     return m, losses
 
 
-def evaluate(m, seeds, warmup=60, steps=40, tol=0.005, max_iters=64, n_side=(64, 64, 64), box=(0, 0, 0, 6.4, 4.0, 3.2), log=print):
-    res = {"plain": [], "learned": []}
+def error_curves(s: Solver, plain_iters, learned_passes):
+    """From the solver's current state: {("plain", I) | ("learned", p): (mean |C|, compression,
+    kept)} after one step; the state is restored afterwards.  s needs FLAG_DENSITY_STATS and, for
+    learned passes, an attached network."""
+    out = {}
+    s.snapshot()
+    for I in plain_iters:
+        s.restore()
+        s.step_adaptive(False, -1.0, I, I)
+        a, c, _ = s.density_stats()
+        out[("plain", I)] = (a, c, None)
+    for p in learned_passes:
+        s.restore()
+        s.step_adaptive(True, -1.0, p, p)
+        a, c, _ = s.density_stats()
+        out[("learned", p)] = (a, c, bool(s.last_guess_info()[2]))
+    s.restore()
+    return out
+
+
+def timed(sc, warmup, steps, iters=None, mlp=None, passes=0):
+    over = {} if iters is None else {"iterations": iters}
+    s = Solver.from_scene(sc, flags=FLAG_GRAPH, **over)
+    s.advance(warmup)
+    if mlp is not None:
+        s.attach_mlp(mlp)
+        s.set_learned(passes)
+    s.advance(3)
+    s.synchronize()
+    ms = s.time_steps(steps) / steps
+    s.close()
+    return ms
+
+
+def evaluate(m, seeds, warmup=60, nstates=6, gap=8, n_side=(64, 64, 64), box=(0, 0, 0, 6.4, 4.0, 3.2),
+             passes=(1, 2, 3, 4), plain_iters=(1, 2, 3, 4, 6, 8, 12, 16, 24, 32), time_steps=100, log=print):
+    rows = []
     for seed in seeds:
         sc = scenes.dam_break(n_side, box, seed=seed)
-        for mode in ("plain", "learned"):
-            s = Solver.from_scene(sc, flags=FLAG_DENSITY_STATS)
-            # common warm-up: plain fixed-iteration steps with the multiplier accumulated
-            for _ in range(warmup):
-                s.step_adaptive(False, 0.0, 4, 4)
-            if mode == "learned":
-                s.attach_mlp(m)
-            s.synchronize()
-            its, resid, kept, final = [], [], 0, []
-            t0 = time.perf_counter()
-            for _ in range(steps):
-                it, r = s.step_adaptive(mode == "learned", tol, 1, max_iters)
-                its.append(it)
-                resid.append(r)
-                if mode == "learned":
-                    kept += int(s.last_guess_info()[2])
-            s.synchronize()
-            wall = time.perf_counter() - t0
-            st = s.density_stats()
-            res[mode].append(
-                dict(
-                    seed=seed,
-                    mean_iters=float(np.mean(its)),
-                    reached=float(np.mean(np.array(resid) <= tol)),
-                    ms_per_step=1000.0 * wall / steps,
-                    guess_kept=kept / steps if mode == "learned" else None,
-                    final_mean_abs_err=st[0],
-                    final_mean_compression=st[1],
-                )
-            )
-            s.close()
-            log(f"  seed {seed} {mode:7s}: {np.mean(its):5.1f} iters/step, reached tol {np.mean(np.array(resid) <= tol):.2f}, {1000.0 * wall / steps:.2f} ms/step")
-    summ = {}
-    for mode in res:
-        summ[mode] = {k: float(np.mean([r[k] for r in res[mode]])) for k in ("mean_iters", "reached", "ms_per_step", "final_mean_compression")}
-    summ["time_saving"] = 1.0 - summ["learned"]["ms_per_step"] / summ["plain"]["ms_per_step"]
-    return res, summ
+        s = Solver.from_scene(sc, flags=FLAG_DENSITY_STATS | FLAG_GRAPH)
+        s.advance(warmup)
+        s.attach_mlp(m)
+        for k in range(nstates):
+            s.advance(gap)
+            cur = error_curves(s, plain_iters, passes)
+            rows.append(dict(seed=seed, state=k, curves={f"{a}_{b}": v for (a, b), v in cur.items()}))
+        s.close()
+        log(f"  seed {seed}: {nstates} states")
+    tgt_c = np.array([r["curves"][f"plain_{TARGET_ITERS}"][1] for r in rows])
+    tgt_a = np.array([r["curves"][f"plain_{TARGET_ITERS}"][0] for r in rows])
+    plain_c = {I: np.array([r["curves"][f"plain_{I}"][1] for r in rows]) for I in plain_iters}
+    plain_a = {I: np.array([r["curves"][f"plain_{I}"][0] for r in rows]) for I in plain_iters}
+    summary = {"target": f"compression after {TARGET_ITERS} plain iterations", "relax": RELAX, "passes": {}, "plain": {}}
+    for I in plain_iters:
+        summary["plain"][I] = {"mean_compression": float(plain_c[I].mean()), "mean_abs_err": float(plain_a[I].mean())}
+    best = None
+    for p in passes:
+        lc = np.array([r["curves"][f"learned_{p}"][1] for r in rows])
+        la = np.array([r["curves"][f"learned_{p}"][0] for r in rows])
+        kept = np.array([r["curves"][f"learned_{p}"][2] for r in rows])
+        # plain iterations that reach the learned step's compression, per state
+        eq = [min([I for I in plain_iters if plain_c[I][i] <= lc[i]], default=max(plain_iters) + 1) for i in range(len(rows))]
+        reach = float(np.mean(lc <= tgt_c))
+        summary["passes"][p] = {
+            "mean_compression": float(lc.mean()),
+            "mean_abs_err": float(la.mean()),
+            "reached_target": reach,
+            "reached_target_abs": float(np.mean(la <= tgt_a)),
+            "kept": float(kept.mean()),
+            "plain_iters_equivalent_median": float(np.median(eq)),
+        }
+        if best is None and reach == 1.0:
+            best = p
+    summary["learned_passes_to_target"] = best
+    # the stated absolute tolerance: 0.5% mean compression
+    summary["plain_reaches_0.005"] = {I: float(np.mean(plain_c[I] <= 0.005)) for I in plain_iters}
+    sc = scenes.dam_break(n_side, box, seed=seeds[0])
+    ms_plain = timed(sc, warmup, time_steps, iters=TARGET_ITERS)
+    summary["ms_per_step_plain_target"] = ms_plain
+    summary["ms_per_step_learned"] = {p: timed(sc, warmup, time_steps, mlp=m, passes=p) for p in passes}
+    if best is not None:
+        ms_l = summary["ms_per_step_learned"][best]
+        summary["ms_per_step_learned_to_target"] = ms_l
+        summary["time_saving"] = 1.0 - ms_l / ms_plain
+    else:
+        summary["time_saving"] = None
+    return rows, summary
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--quick", action="store_true", help="small scenes and short runs")
-    ap.add_argument("--tol", type=float, default=0.005)
     ap.add_argument("--out", default=None)
     ap.add_argument("--train-steps", type=int, default=3000)
     args = ap.parse_args()
     if args.quick:
         n_side, box = (32, 32, 32), (0, 0, 0, 3.2, 2.0, 1.6)
-        train_seeds, eval_seeds, rec_steps, warm, ev = range(100, 104), range(200, 202), 100, 30, 20
+        train_seeds, eval_seeds, rec_steps, stride, warm = range(100, 104), (200, 201), 150, 8, 40
     else:
         n_side, box = (64, 64, 64), (0, 0, 0, 6.4, 4.0, 3.2)
-        train_seeds, eval_seeds, rec_steps, warm, ev = range(100, 108), range(200, 204), 300, 60, 40
+        train_seeds, eval_seeds, rec_steps, stride, warm = range(100, 108), (200, 201, 202, 203), 300, 64, 60
     print(f"record: {len(train_seeds)} seeds x {rec_steps} steps, {n_side} lattice", flush=True)
-    ds, rows = record_dataset(train_seeds, rec_steps, n_side, box)
+    ds, rows = record_dataset(train_seeds, rec_steps, n_side, box, stride=stride)
     mean, inv, ym, ys, nvalid = normalise(ds, rows)
+    ds_close = ds.close
     print(f"dataset: {nvalid} samples; target mean {ym:.4g} std {ys:.4g}", flush=True)
     m, losses = train(ds, nvalid, mean, inv, ym, ys, steps=args.train_steps)
-    res, summ = evaluate(m, eval_seeds, warm, ev, args.tol, 64, n_side, box)
-    out = dict(tol=args.tol, samples=nvalid, loss_start=float(losses[:20].mean()), loss_end=float(losses[-100:].mean()), summary=summ, runs=res)
+    ds_close()
+    rows_, summ = evaluate(m, list(eval_seeds), warm, n_side=n_side, box=box)
+    out = dict(samples=nvalid, loss_start=float(losses[:20].mean()), loss_end=float(losses[-100:].mean()), summary=summ, states=rows_)
     print(json.dumps(summ), flush=True)
     if args.out:
         with open(args.out, "w") as f:
-            json.dump(out, f, indent=1)
+            json.dump(out, f, indent=1, default=str)
 
 
 if __name__ == "__main__":

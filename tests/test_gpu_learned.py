@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 from oracle_lib import MLP_SLICES, Oracle, mlp_adam, mlp_forward, mlp_loss_grad
-from pbf_amd import FLAG_DEBUG, DeviceDataset, Mlp, Solver, scenes
+from pbf_amd import FLAG_DEBUG, FLAG_DENSITY_STATS, FLAG_GRAPH, DeviceDataset, Mlp, PBFError, Solver, learned, scenes
 
 pytestmark = pytest.mark.gpu
 
@@ -122,22 +122,99 @@ def test_garbage_network_is_rejected_and_reproduces_plain():
 
 
 def test_record_step_features_and_targets():
-    """Features of the neighbour pass and the one-iteration target against the oracle."""
+    """Features of the neighbour pass and the one-iteration target against the oracle, over two
+    steps (the second step's feature 7 is the first step's iteration-1 lambda)."""
     print("[case] record step", flush=True)
     sc = scenes.dam_break((16, 16, 16), (0, 0, 0, 1.6, 1.6, 1.6), seed=21)
     s = Solver.from_scene(sc, flags=FLAG_DEBUG)
     o = Oracle.from_scene(sc, iterations=1)
-    ds = DeviceDataset(sc.n)
-    s.record_step(1, ds, base=0, stride=1, phase=0)
-    o.step(1)
-    x, t = ds.host(sc.n)
-    nbr, cnt = o.neighbors()
-    lam1 = o.fields()[0]
+    ds = DeviceDataset(2 * sc.n)
     h = np.float32(sc.params["h"])
     ih = np.float32(1.0) / h
-    assert np.array_equal(x[:, 1], cnt.astype(np.float32))
-    assert np.array_equal(x[:, 15], lam1 * (ih * ih))
     inv_h2 = np.float32(1.0) / (h * h)
-    assert np.array_equal(t, lam1 * inv_h2)
-    assert (x[:, 8] >= 0).all() and (x[:, 8] <= 4).all()
-    np.testing.assert_array_equal(x[:, 8], np.minimum(np.min(x[:, 9:15], axis=1), 4.0))
+    prev = np.zeros(sc.n, np.float32)
+    for k in range(2):
+        s.record_step(1, ds, base=k * sc.n, stride=1, phase=0)
+        o.step(1)
+        x, t = ds.host(2 * sc.n)
+        x, t = x[k * sc.n:(k + 1) * sc.n], t[k * sc.n:(k + 1) * sc.n]
+        nbr, cnt = o.neighbors()
+        lam1 = o.fields()[0]
+        assert np.array_equal(x[:, 1], cnt.astype(np.float32))
+        assert np.array_equal(x[:, 15], lam1 * (ih * ih))
+        assert np.array_equal(x[:, 7], prev * (ih * ih))
+        assert np.array_equal(t, lam1 * inv_h2)
+        assert (x[:, 8] >= 0).all() and (x[:, 8] <= 4).all()
+        np.testing.assert_array_equal(x[:, 8], np.minimum(np.min(x[:, 9:15], axis=1), 4.0))
+        prev = lam1
+
+
+def test_snapshot_restore_is_exact():
+    print("[case] snapshot / restore", flush=True)
+    sc = scenes.config_cpu_32k()
+    s = Solver.from_scene(sc)
+    s.advance(3)
+    s.snapshot()
+    s.advance(2)
+    p1, v1 = s.get_state()
+    s.restore()
+    s.advance(2)
+    p2, v2 = s.get_state()
+    assert np.array_equal(p1.view(np.uint32), p2.view(np.uint32))
+    assert np.array_equal(v1.view(np.uint32), v2.view(np.uint32))
+
+
+def test_set_learned_needs_a_network():
+    s = Solver.from_scene(scenes.dam_break((8, 8, 8)))
+    with pytest.raises(PBFError):
+        s.set_learned(2)
+
+
+@pytest.mark.parametrize("passes", [1, 3])
+def test_rejected_learned_step_is_the_plain_step(passes):
+    """A network whose pass scatters the fluid is rejected on the device, and the graph-replayed
+    learned step then equals the plain step with the same number of iterations, bit for bit."""
+    print(f"[case] device-side fallback, {passes} passes", flush=True)
+    sc = scenes.dam_break((32, 32, 32), (0, 0, 0, 3.2, 2.0, 1.6), seed=5)
+    plain = Solver.from_scene(sc, flags=FLAG_GRAPH, iterations=passes)
+    lrn = Solver.from_scene(sc, flags=FLAG_GRAPH, iterations=passes)
+    m = Mlp(seed=11)
+    m.set_params(random_params(11, scale=3.0))
+    m.set_normalization(np.zeros(16), np.ones(16), 1e3, 0.0)
+    lrn.attach_mlp(m)
+    lrn.set_learned(passes)
+    plain.advance(6)
+    lrn.advance(6)
+    assert not lrn.last_guess_info()[2]
+    pa, va = plain.get_state()
+    pb, vb = lrn.get_state()
+    assert np.array_equal(pa.view(np.uint32), pb.view(np.uint32))
+    assert np.array_equal(va.view(np.uint32), vb.view(np.uint32))
+
+
+def test_trained_network_gain():
+    """Train on small dam breaks (32^3 lattice in a 3.2 x 2.0 x 1.6 m box), then from held-out
+    states: the learned step with 3 passes (the network's pass + 2 plain iterations) reaches the
+    compression that 8 plain iterations reach, and with 2 passes it beats the plain step with the
+    same 2 iterations by more than 10%."""
+    print("[case] trained network gain", flush=True)
+    n_side, box = (32, 32, 32), (0, 0, 0, 3.2, 2.0, 1.6)
+    ds, rows = learned.record_dataset(range(100, 104), 100, n_side, box, stride=4, log=lambda *a: None)
+    mean, inv, ym, ys, nvalid = learned.normalise(ds, rows)
+    m, losses = learned.train(ds, nvalid, mean, inv, ym, ys, steps=3000, log=lambda *a: None)
+    ds.close()
+    assert losses[-100:].mean() < 0.2 * losses[:20].mean()
+    s = Solver.from_scene(scenes.dam_break(n_side, box, seed=300), flags=FLAG_DENSITY_STATS | FLAG_GRAPH)
+    s.advance(40)
+    s.attach_mlp(m)
+    res = []
+    for _ in range(4):
+        s.advance(8)
+        cur = learned.error_curves(s, (2, 8), (2, 3))
+        row = (cur[("plain", 2)][1], cur[("plain", 8)][1], cur[("learned", 2)][1], cur[("learned", 3)][1])
+        res.append(row)
+        print("  compression plain-2 %.5f plain-8 %.5f learned-2 %.5f learned-3 %.5f" % row, flush=True)
+        assert cur[("learned", 2)][2] and cur[("learned", 3)][2], "the trained network's pass is kept"
+    res = np.array(res)
+    assert (res[:, 3] <= res[:, 1]).all(), "learned (3 passes) reaches the 8-iteration compression"
+    assert (res[:, 2] < 0.9 * res[:, 0]).all(), "learned (2 passes) beats 2 plain iterations"
