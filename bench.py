@@ -42,7 +42,20 @@ WORKLOADS = {
     "dam_break_262k": scenes.config_262k,
     "cpu_32k": scenes.config_cpu_32k,
     "double_dam_break_1m": scenes.config_1m,
+    "batch_24_scenes": scenes.batch_config,
 }
+
+
+def sequential_ms(scene_list, warmup, steps):
+    """Sum over the scenes of the graph-replayed step time of each scene alone."""
+    tot = 0.0
+    for sc in scene_list:
+        s = Solver.from_scene(sc, flags=FLAG_GRAPH)
+        s.advance(warmup)
+        s.synchronize()
+        tot += s.time_steps(steps) / steps
+        s.close()
+    return tot
 
 
 def algorithmic_bytes(stage: str, n: int, ncells: int) -> float:
@@ -163,6 +176,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--copy-mb", type=int, default=512, help="size of the calibration copy kernel (0: skip)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start leg")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -173,16 +187,28 @@ def main():
         sys.exit(2)
 
     copy_gbs = copy_bandwidth(args.copy_mb) if args.copy_mb > 0 else None
-    scene = WORKLOADS[args.workload]()
     flags = 0 if args.no_graph else FLAG_GRAPH
-    s = Solver.from_scene(scene, flags=flags)
-    n = scene.n
-    ncells = int(np.prod(scene.grid_dims))
+    batch = None
+    if args.workload == "batch_24_scenes":
+        batch = WORKLOADS[args.workload]()
+        s = Solver.batch(batch, flags=flags)
+        scene = max(batch, key=lambda sc: sc.n)  # the CPU baseline's sample
+        n, ncells = s.n, s.ncells
+        name = f"batch of {len(batch)} dam breaks ({n} particles, {min(sc.n for sc in batch)}-{max(sc.n for sc in batch)} per scene)"
+    else:
+        scene = WORKLOADS[args.workload]()
+        s = Solver.from_scene(scene, flags=flags)
+        n = scene.n
+        ncells = int(np.prod(scene.grid_dims))
+        name = scene.name
     iters = int(scene.params["iterations"])
 
     if args.warmup > 0:
         s.advance(args.warmup)
         s.synchronize()
+    tuning, tune_ms = s.get_tuning(), None
+    if not args.no_autotune:
+        tuning, tune_ms = s.autotune(10)
     ms_total = s.time_steps(args.steps)
     ms_step = ms_total / max(args.steps, 1)
     value = n / (ms_step * 1e-3)
@@ -191,15 +217,18 @@ def main():
     acc = {}
     cnt = {}
     for _ in range(max(1, args.profile_steps)):
-        for name, ms in s.profile_step():
-            acc[name] = acc.get(name, 0.0) + ms
-            cnt[name] = cnt.get(name, 0) + 1
+        for stage, ms in s.profile_step():
+            acc[stage] = acc.get(stage, 0.0) + ms
+            cnt[stage] = cnt.get(stage, 0) + 1
     launches_per_step = {k: cnt[k] / max(1, args.profile_steps) for k in cnt}
     avg_ms = {k: acc[k] / cnt[k] for k in acc}  # per launch
     per_step_ms = {k: avg_ms[k] * launches_per_step[k] for k in avg_ms}
     dom = max(per_step_ms, key=per_step_ms.get)
     dom_bytes = algorithmic_bytes(dom, n, ncells)
-    achieved = dom_bytes / (avg_ms[dom] * 1e-3) / 1e9
+    # launch duration: back-to-back replays between two events (no per-launch event overhead)
+    # where the stage can be replayed, else the per-launch events of the profile steps
+    launch_ms = s.time_stage(dom, 50) if dom in Solver.STAGES else avg_ms[dom]
+    achieved = dom_bytes / (launch_ms * 1e-3) / 1e9
     sort_stage = ["predict_hash", "scan", "scatter_rank_reorder"]
     sort_ms = sum(per_step_ms.get(k, 0.0) for k in sort_stage)
     sort_bytes = float(n * 96 + 12 * ncells)
@@ -207,6 +236,10 @@ def main():
     pos, vel = s.get_state()
     finite = bool(np.isfinite(pos).all() and np.isfinite(vel).all())
     s.close()
+    seq = None
+    if batch is not None:
+        seq_ms = sequential_ms(batch, args.warmup, max(20, args.steps // 5))
+        seq = {"ms_per_step_sum_of_scenes_alone": seq_ms, "batched_speedup": seq_ms / ms_step}
 
     learned_res = None
     if not args.no_learned and args.workload == "dam_break_262k":
@@ -233,13 +266,14 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded SplitMix64 dam-break lattice)",
         "config": {
-            "workload": scene.name,
+            "workload": name,
             "particles": n,
-            "grid": list(scene.grid_dims),
+            "grid": list(scene.grid_dims) if batch is None else "one per scene",
             "iterations": iters,
             "dt": scene.params["dt"],
             "graph": not args.no_graph,
             "parallelism": "single GPU",
+            "tuning": {"block": tuning[0], "grid": tuning[1], "xcd_map": tuning[2], "autotuned": not args.no_autotune},
         },
         "roofline": {
             "bound": "hbm",
@@ -250,7 +284,8 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(args.workload, dom),
             "algorithmic_bytes_per_launch": dom_bytes,
-            "avg_launch_ms": avg_ms[dom],
+            "avg_launch_ms": launch_ms,
+            "avg_launch_ms_events_per_launch": avg_ms[dom],
         },
         "stages_ms": {k: round(v, 5) for k, v in per_step_ms.items()},
         "step_roofline_frac": step_bytes(n, ncells, iters) / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9),
@@ -259,6 +294,7 @@ def main():
         "state_finite": finite,
         "cpu_baseline": cpu,
         "learned": learned_res,
+        "batch_vs_sequential": seq,
     }
     print(json.dumps(out), flush=True)
 

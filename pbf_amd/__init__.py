@@ -165,6 +165,7 @@ def lib() -> C.CDLL:
         "pbf_step_adaptive": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_float, C.c_int32, C.c_int32, _I, _F]),
         "pbf_last_guess_info": (C.c_int, [_VP, _F]),
         "pbf_set_learned": (C.c_int, [_VP, C.c_int32]),
+        "pbf_time_stage": (C.c_int, [_VP, C.c_int32, C.c_int32, _F]),
         "pbf_snapshot": (C.c_int, [_VP, C.c_int32]),
         "pbf_create_batch": (C.c_int, [C.POINTER(SolverParameter), C.c_int32, _I, _F, _F, _I, _F, _F, C.POINTER(_VP)]),
         "pbf_set_tuning": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32]),
@@ -359,6 +360,15 @@ class Solver:
     def time_steps(self, nsteps: int) -> float:
         ms = np.zeros(1, np.float32)
         _check(lib().pbf_time_steps(self._h, int(nsteps), _fp(ms)))
+        return float(ms[0])
+
+    STAGES = {"neighbors_lambda": 0, "lambda": 1, "delta_apply": 2}
+
+    def time_stage(self, stage: str, reps: int = 50) -> float:
+        """ms per launch of one constraint kernel ("neighbors_lambda", "lambda", "delta_apply"),
+        replayed back to back on the last step's state (state unchanged)."""
+        ms = np.zeros(1, np.float32)
+        _check(lib().pbf_time_stage(self._h, self.STAGES[stage], int(reps), _fp(ms)))
         return float(ms[0])
 
     def profile_step(self):

@@ -780,6 +780,39 @@ int pbf_profile_step(pbf_solver* s, float* ms, const char** names, int32_t max) 
 }
 
 
+// Average duration of one launch of a constraint kernel (0: neighbour build + lambda 1,
+// 1: lambda, 2: delta-p + apply), replayed reps times back to back on the state of the last step
+// between two HIP events on the solver's stream.  The replays write only buffers that the next
+// step rebuilds (lists, lambda in pos.w, the free position buffer), so the state is unchanged.
+int pbf_time_stage(pbf_solver* s, int32_t stage, int32_t reps, float* ms) {
+    if (!s || !ms || reps < 1 || stage < 0 || stage > 2) return set_error(PBF_ERR_ARG, "bad argument");
+    if (s->steps_done < 1 || s->n == 0) return set_error(PBF_ERR_STATE, "run a step first");
+    HIP_TRY(hipSetDevice(s->device));
+    const StepConsts K = make_consts(s);
+    StepBuffers& B = s->B;
+    const int a = s->pc;
+    const LearnArgs L0{};
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, s->stream));
+    for (int r = 0; r < reps; ++r) {
+        if (stage == 0) launch_neighbors_lambda(s->stream, K, s->geo, s->tune, B.pos[a], B, nullptr, L0);
+        if (stage == 1) launch_lambda(s->stream, K, s->tune, B.pos[a], B, L0);
+        if (stage == 2) launch_delta_apply(s->stream, K, s->geo, s->tune, B.pos[a], B.pos[a ^ 1], B.vel[s->vc], B, 0, 1, L0);
+    }
+    HIP_TRY(hipEventRecord(e1, s->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float t = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIP_TRY(hipGetLastError());
+    s->lam_last = -1;
+    *ms = t / (float)reps;
+    return PBF_OK;
+}
+
 // Times every launch shape (workgroup size x persistent grid x XCD mapping) over `steps`
 // graph-replayed steps each and keeps the fastest.  Results do not depend on the launch shape,
 // and the state is restored afterwards, so tuning does not advance the simulation.  best[3]

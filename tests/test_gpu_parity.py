@@ -231,3 +231,23 @@ def test_20_steps_262k_bitexact():
     op, ov = o.get_state()
     assert_bitexact(gp, op, "positions")
     assert_bitexact(gv, ov, "velocities")
+
+
+@pytest.mark.timeout(900)
+def test_1000_steps_density_statistics_262k():
+    """The 262k dam break over 1000 steps (the oracle takes about a minute on 16 host threads):
+    density-error statistics within 1% every 250 steps, and the final state bit for bit."""
+    print("[case] 262k x 1000 steps", flush=True)
+    g, o = pair(scenes.config_262k(), flags=FLAG_DENSITY_STATS)
+    for k in range(4):
+        g.advance(250)
+        o.step(250)
+        st = g.density_stats()
+        err = o.fields()[2].astype(np.float64)
+        ref = (np.abs(err).mean(), np.maximum(err, 0).mean(), err.max())
+        np.testing.assert_allclose(st, ref, rtol=1e-2, err_msg=f"step {250 * (k + 1)}")
+        print(f"  step {250 * (k + 1)}: mean |err| {st[0]:.5f} compression {st[1]:.5f} max {st[2]:.4f}", flush=True)
+    gp, gv = g.get_state()
+    op, ov = o.get_state()
+    assert_bitexact(gp, op, "positions after 1000 steps")
+    assert_bitexact(gv, ov, "velocities after 1000 steps")
