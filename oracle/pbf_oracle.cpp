@@ -12,8 +12,9 @@
 //   * pow() of the reference is replaced by repeated multiplication for integer exponents;
 //   * the boundary jitter's sin() is the self-written pbf_sin below (Cody-Waite reduction +
 //     truncated Taylor series), not libm's, so it is identical on both sides;
-//   * the spiky gradient is evaluated as r * ((cs*(h-|r|)^2)/|r|) -- the reference
-//     (cuh:105-110) evaluates (cs*(h-|r|)^2) * (r/|r|); the two agree to rounding;
+//   * the spiky gradient is evaluated as r * (cs*(h-|r|)^2 * rsqrt(r.r)) with |r| = r.r *
+//     rsqrt(r.r) and rsqrt from a bit-pattern seed and three Newton steps (pbf_rsqrt) -- the
+//     reference (cuh:105-110) evaluates (cs*(h-|r|)^2) * (r/|r|) with sqrt; they agree to ~2e-7;
 //   * s_corr's ratio W(r)/W(dq) (cuh:143) is evaluated as W(r) * (1/W(dq));
 //   * particles inside one cell are ordered by ascending particle index (the reference's
 //     atomicSub scatter, cu:112-122, leaves that order undefined).
@@ -143,11 +144,29 @@ inline float poly6(float r2, const Consts& K) {
     return K.c6 * (d * d * d);
 }
 
+// 1/sqrt(x): Lomont's refined bit-pattern seed 0x5f375a86 (C. Lomont, "Fast Inverse Square
+// Root", 2003) and three Newton steps y <- y (1.5 - ((x/2) y) y), plain IEEE fp32 (the kernels
+// use the same steps).
+inline float pbf_rsqrt(float x) {
+    uint32_t i;
+    std::memcpy(&i, &x, sizeof(i));
+    i = 0x5f375a86u - (i >> 1);
+    float y;
+    std::memcpy(&y, &i, sizeof(y));
+    const float hx = 0.5f * x;
+    y = y * (1.5f - (hx * y) * y);
+    y = y * (1.5f - (hx * y) * y);
+    y = y * (1.5f - (hx * y) * y);
+    return y;
+}
+
+// WspikyGrad, cuh:105-110: |r| and 1/|r| from pbf_rsqrt(r.r).
 inline V3 spiky_grad(const V3& r, float r2, const Consts& K) {
-    float len = std::sqrt(r2);
+    const float inv = pbf_rsqrt(r2);
+    const float len = r2 * inv;
     if (len > K.h || len < K.rmin) return {0.0f, 0.0f, 0.0f};
     float hl = K.h - len;
-    float t = (K.cs * (hl * hl)) / len;
+    float t = (K.cs * (hl * hl)) * inv;
     return {r.x * t, r.y * t, r.z * t};
 }
 

@@ -3,6 +3,8 @@
 // arithmetic is fixed so that a CPU build of the same operations gives the same bits:
 //   * fp32 +,-,*,/ and sqrt only (correctly rounded; the library is built with
 //     -ffp-contract=off and -fhip-fp32-correctly-rounded-divide-sqrt), left to right;
+//   * 1/|r| of the spiky gradient from pbf_rsqrt (bit-pattern seed + Newton steps in those
+//     operations), not from sqrt and a division;
 //   * integer powers by repeated multiplication;
 //   * the jitter's sin is pbf_sin below, not the device libm's.
 #pragma once
@@ -95,12 +97,25 @@ __device__ __forceinline__ float poly6(float r2, const StepConsts& K) {
     return r2 > K.h2 ? 0.0f : w;
 }
 
+// 1/sqrt(x): Lomont's refined bit-pattern seed 0x5f375a86 (C. Lomont, "Fast Inverse Square
+// Root", 2003) and three Newton steps y <- y (1.5 - ((x/2) y) y), in IEEE fp32 with no
+// contraction, so the oracle computes the same bits.  Relative error ~2e-7 for normal x > 0.
+__device__ __forceinline__ float pbf_rsqrt(float x) {
+    float y = __uint_as_float(0x5f375a86u - (__float_as_uint(x) >> 1));
+    const float hx = 0.5f * x;
+    y = y * (1.5f - (hx * y) * y);
+    y = y * (1.5f - (hx * y) * y);
+    y = y * (1.5f - (hx * y) * y);
+    return y;
+}
+
 // WspikyGrad, cuh:105-110, returned as the scalar t with grad = r * t (evaluated, then
-// selected: no branch).
+// selected: no branch): |r| = r.r * rsqrt(r.r) and t = cs (h - |r|)^2 * rsqrt(r.r).
 __device__ __forceinline__ float spiky_scale(float r2, const StepConsts& K) {
-    float len = sqrtf(r2);
-    float hl = K.h - len;
-    float t = (K.cs * (hl * hl)) / len;
+    const float inv = pbf_rsqrt(r2);
+    const float len = r2 * inv;
+    const float hl = K.h - len;
+    const float t = (K.cs * (hl * hl)) * inv;
     return (len > K.h || len < K.rmin) ? 0.0f : t;
 }
 

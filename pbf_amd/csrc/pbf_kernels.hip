@@ -289,6 +289,14 @@ __device__ __forceinline__ size_t nbr_index(int s, int j, int nstride) {
     return ((size_t)(j >> 3) * (size_t)nstride + (size_t)s) * 8 + (size_t)(j & 7);
 }
 
+
+// Neighbour loops fetch in groups of kGroup: all list entries of the group, then all gathered
+// operands, then the group is processed in list order (the summation order of the reference's
+// loop).  Entries past cnt re-read the last valid entry and are skipped, so every fetch is in
+// bounds and branch-free.  kGroup independent loads in flight per lane instead of one.
+constexpr int kGroup = 8;
+
+// Calls f(q, x_q) for the neighbours j = 0..cnt-1 of slot s in order, x_q = pos[q].
 // The 8 list entries of group j0 (a multiple of 8) of slot s; entries past cnt are replaced by
 // s itself so every gather stays in bounds.
 __device__ __forceinline__ void load_group(const uint32_t* __restrict__ nbr, int s, int j0, int cnt, int nstride,
@@ -308,13 +316,6 @@ __device__ __forceinline__ void load_group(const uint32_t* __restrict__ nbr, int
         if (j0 + u >= cnt) q[u] = (uint32_t)s;
 }
 
-// Neighbour loops fetch in groups of kGroup: all list entries of the group, then all gathered
-// operands, then the group is processed in list order (the summation order of the reference's
-// loop).  Entries past cnt re-read the last valid entry and are skipped, so every fetch is in
-// bounds and branch-free.  kGroup independent loads in flight per lane instead of one.
-constexpr int kGroup = 8;
-
-// Calls f(q, x_q) for the neighbours j = 0..cnt-1 of slot s in order, x_q = pos[q].
 template <class F>
 __device__ __forceinline__ void for_neighbors(const uint32_t* __restrict__ nbr, int s, int nstride, int cnt,
                                               const float4* __restrict__ pos, F&& f) {
@@ -351,35 +352,35 @@ __device__ void block_sum2(float c, float d, float* __restrict__ partial) {
     }
 }
 
-// buildNeighborhood (cu:124-154) fused with the first calculateLambda: the 27 cells in z, y, x
-// order (the three x-neighbours of a row are one contiguous slot run), self excluded,
-// r.r <= h*h, at most cap hits.  Lambda of iteration 1 goes to pos.w and to lam1 (kept for the
-// learned step's fallback and, through vel.w, as the next step's feature).  With L.feat the raw
-// warm-start features of the particle are written as well (kNumFeatures floats, see
-// pbf_kernels.h); with L.resid the workgroup sums of max(C, 0) and |C| at the predicted positions.
+// buildNeighborhood (cu:124-154): the 27 cells in z, y, x order (the three x-neighbours of a row
+// are one contiguous slot run), self excluded, r.r <= h*h, at most cap hits, in that order.
+// Eight lanes work on one particle: they test 8 consecutive candidates of a run at a time (one
+// 128-byte load per subgroup) and a ballot gives each hit its place in the list, so the list
+// keeps the candidates' order.  Cell culling: a neighbour cell whose box lies farther than h
+// from the particle cannot hold a neighbour and is left out of the run (the x-cells that remain
+// are contiguous, so the order is unchanged).  Distances are taken to the particle's own cell
+// faces less a margin of 1e-3 h, which covers the rounding of the cell index.
+constexpr int kSub = 8;  // lanes per particle in the search
+
 template <int BS>
-__global__ __launch_bounds__(BS) void k_neighbors_lambda(StepConsts K, const SceneGeom* __restrict__ geoms,
-                                                         const uint8_t* __restrict__ sid, float4* __restrict__ pos,
-                                                         const uint32_t* __restrict__ cstart,
-                                                         uint32_t* __restrict__ nbr, uint8_t* __restrict__ ncnt,
-                                                         float* __restrict__ lam1, float* __restrict__ dbg_lam,
-                                                         LearnArgs L, int xcd_map) {
-    const BlockRange R((K.n + BS - 1) / BS, xcd_map);
-    float comp = 0.0f, dev = 0.0f;
+__global__ __launch_bounds__(BS) void k_search(StepConsts K, const SceneGeom* __restrict__ geoms,
+                                               const uint8_t* __restrict__ sid, const float4* __restrict__ pos,
+                                               const uint32_t* __restrict__ cstart, uint32_t* __restrict__ nbr,
+                                               uint8_t* __restrict__ ncnt, int xcd_map) {
+    constexpr int PPB = BS / kSub;  // particles per workgroup pass
+    const BlockRange R((K.n + PPB - 1) / PPB, xcd_map);
+    const int lane8 = threadIdx.x & (kSub - 1);
+    const int wsub = (threadIdx.x & 63) / kSub;  // subgroup inside the wavefront
+    const uint32_t below = (1u << lane8) - 1u;
     for (int blk = R.first; blk < R.last; ++blk) {
-        const int s = blk * BS + threadIdx.x;
-        if (s >= K.n) continue;
+        const int s = blk * PPB + (int)threadIdx.x / kSub;
+        if (s >= K.n) continue;  // whole subgroups leave together
         const SceneGeom G = geom_of(K, geoms, sid, s);
         const float4 xi = pos[s];
-        int d0 = dim_of(xi.x, G.gmin[0], K.invH, G.dims[0]);
-        int d1 = dim_of(xi.y, G.gmin[1], K.invH, G.dims[1]);
-        int d2 = dim_of(xi.z, G.gmin[2], K.invH, G.dims[2]);
+        const int d0 = dim_of(xi.x, G.gmin[0], K.invH, G.dims[0]);
+        const int d1 = dim_of(xi.y, G.gmin[1], K.invH, G.dims[1]);
+        const int d2 = dim_of(xi.z, G.gmin[2], K.invH, G.dims[2]);
         const int xlo = max(d0 - 1, 0), xhi = min(d0 + 1, G.dims[0] - 1);
-        // Cell culling: a neighbour cell whose box lies farther than h from the particle cannot
-        // hold a neighbour, so it is left out of the run (the x-cells that remain are contiguous,
-        // and the list keeps the reference order).  Distances to the particle's own cell faces,
-        // less a margin of 1e-3 h that covers the rounding of the cell index, so that no cell
-        // holding a particle within h is ever culled.  Only for particles inside the grid.
         const bool inside = d0 >= 0 && d0 < G.dims[0] && d1 >= 0 && d1 < G.dims[1] && d2 >= 0 && d2 < G.dims[2];
         float fl[3] = {0.0f, 0.0f, 0.0f}, fh[3] = {0.0f, 0.0f, 0.0f};
         if (inside) {
@@ -393,62 +394,73 @@ __global__ __launch_bounds__(BS) void k_neighbors_lambda(StepConsts K, const Sce
                 fh[a] = fmaxf(lo + K.h - p3[a] - margin, 0.0f);
             }
         }
-        // Search: hits go to a register block of 8 list entries, written as 2 x uint4 when full.
-        // The lambda accumulation runs afterwards over the finished list (same order), so the
-        // search loop, whose candidate slots diverge across the wavefront, stays light.
-        int cnt = 0;
-        uint32_t blk8[kGroup];
+        // The 9 row runs first (their bounds are independent loads), empty when out of the grid
+        // or culled.
+        uint32_t rb[9], re[9];
 #pragma unroll
-        for (int u = 0; u < kGroup; ++u) blk8[u] = (uint32_t)s;
-        for (int dz = -1; dz <= 1; ++dz) {
-            int cz = d2 + dz;
-            if (cz < 0 || cz >= G.dims[2]) continue;
+        for (int r = 0; r < 9; ++r) {
+            const int dz = r / 3 - 1, dy = r % 3 - 1;
+            const int cz = d2 + dz, cy = d1 + dy;
             const float ez = dz < 0 ? fl[2] : (dz > 0 ? fh[2] : 0.0f);
-            for (int dy = -1; dy <= 1; ++dy) {
-                int cy = d1 + dy;
-                if (cy < 0 || cy >= G.dims[1] || xlo > xhi) continue;
-                const float ey = dy < 0 ? fl[1] : (dy > 0 ? fh[1] : 0.0f);
-                const float eyz = ey * ey + ez * ez;
-                int rlo = xlo, rhi = xhi;
-                if (inside) {
-                    if (eyz > K.h2) continue;
-                    if (eyz + fl[0] * fl[0] > K.h2) rlo = d0;
-                    if (eyz + fh[0] * fh[0] > K.h2) rhi = d0;
-                }
-                int row = G.coff + (cz * G.dims[1] + cy) * G.dims[0];
-                uint32_t b = cstart[row + rlo], e = cstart[row + rhi + 1];
-                for (uint32_t q0 = b; q0 < e && cnt < K.cap; q0 += kGroup) {
-                    float4 xs[kGroup];
+            const float ey = dy < 0 ? fl[1] : (dy > 0 ? fh[1] : 0.0f);
+            const float eyz = ey * ey + ez * ez;
+            int rlo = xlo, rhi = xhi;
+            bool use = cz >= 0 && cz < G.dims[2] && cy >= 0 && cy < G.dims[1] && xlo <= xhi;
+            if (inside) {
+                use = use && eyz <= K.h2;
+                if (eyz + fl[0] * fl[0] > K.h2) rlo = d0;
+                if (eyz + fh[0] * fh[0] > K.h2) rhi = d0;
+            }
+            // both indices stay inside the scene's cell table even for an unused run
+            const int row = use ? G.coff + (cz * G.dims[1] + cy) * G.dims[0] : G.coff;
+            const uint32_t lo = cstart[use ? row + rlo : row], hi = cstart[use ? row + rhi + 1 : row];
+            rb[r] = use ? lo : 0u;
+            re[r] = use ? hi : 0u;
+        }
+        // Four chunks of 8 candidates per pass: their loads are issued together.
+        constexpr int kChunks = 4;
+        int cnt = 0;
 #pragma unroll
-                    for (int u = 0; u < kGroup; ++u) xs[u] = pos[min(q0 + (uint32_t)u, e - 1u)];
+        for (int r = 0; r < 9; ++r) {
+            const uint32_t e = re[r];
+            for (uint32_t q0 = rb[r]; q0 < e && cnt < K.cap; q0 += kChunks * kSub) {
+                float4 xs[kChunks];
 #pragma unroll
-                    for (int u = 0; u < kGroup; ++u) {
-                        const uint32_t q = q0 + (uint32_t)u;
-                        if (q >= e || cnt >= K.cap || (int)q == s) continue;
-                        const float4 xj = xs[u];
-                        float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-                        float r2 = rx * rx + ry * ry + rz * rz;
-                        if (r2 <= K.h2) {
-                            const int k = cnt & (kGroup - 1);
+                for (int u = 0; u < kChunks; ++u) xs[u] = pos[min(q0 + (uint32_t)(u * kSub + lane8), e - 1u)];
 #pragma unroll
-                            for (int w = 0; w < kGroup; ++w) blk8[w] = (k == w) ? q : blk8[w];
-                            ++cnt;
-                            if (k == kGroup - 1) {
-                                uint4* o = reinterpret_cast<uint4*>(nbr + nbr_index(s, cnt - kGroup, K.nstride));
-                                o[0] = make_uint4(blk8[0], blk8[1], blk8[2], blk8[3]);
-                                o[1] = make_uint4(blk8[4], blk8[5], blk8[6], blk8[7]);
-                            }
-                        }
-                    }
+                for (int u = 0; u < kChunks; ++u) {
+                    const uint32_t q = q0 + (uint32_t)(u * kSub + lane8);
+                    const float rx = xi.x - xs[u].x, ry = xi.y - xs[u].y, rz = xi.z - xs[u].z;
+                    const bool hit = q < e && (int)q != s && rx * rx + ry * ry + rz * rz <= K.h2;
+                    const uint32_t bits = (uint32_t)(__ballot(hit) >> (kSub * wsub)) & 0xffu;
+                    const int at = cnt + __popc(bits & below);
+                    if (hit && at < K.cap) nbr[nbr_index(s, at, K.nstride)] = q;
+                    cnt += __popc(bits);
                 }
             }
         }
-        if (cnt & (kGroup - 1)) {  // the last, partial block (entries past cnt are never read)
-            uint4* o = reinterpret_cast<uint4*>(nbr + nbr_index(s, cnt & ~(kGroup - 1), K.nstride));
-            o[0] = make_uint4(blk8[0], blk8[1], blk8[2], blk8[3]);
-            o[1] = make_uint4(blk8[4], blk8[5], blk8[6], blk8[7]);
-        }
-        ncnt[s] = (uint8_t)cnt;
+        if (lane8 == 0) ncnt[s] = (uint8_t)min(cnt, K.cap);
+    }
+}
+
+// The first calculateLambda (cu:156-180), over the list just built.  Lambda of iteration 1 goes
+// to pos.w and to lam1 (kept for the learned step's fallback and, through vel.w, as the next
+// step's feature).  With L.feat the raw warm-start features of the particle are written as well
+// (kNumFeatures floats, see pbf_kernels.h); with L.resid the workgroup sums of max(C, 0) and |C|
+// at the predicted positions.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_lambda_first(StepConsts K, const SceneGeom* __restrict__ geoms,
+                                                     const uint8_t* __restrict__ sid, float4* __restrict__ pos,
+                                                     const uint32_t* __restrict__ nbr, const uint8_t* __restrict__ ncnt,
+                                                     float* __restrict__ lam1, float* __restrict__ dbg_lam,
+                                                     LearnArgs L, int xcd_map) {
+    const BlockRange R((K.n + BS - 1) / BS, xcd_map);
+    float comp = 0.0f, dev = 0.0f;
+    for (int blk = R.first; blk < R.last; ++blk) {
+        const int s = blk * BS + threadIdx.x;
+        if (s >= K.n) continue;
+        const float4 xi = pos[s];
+        const int cnt = ncnt[s];
         LambdaAcc A;
         A.init();
         for_neighbors(nbr, s, K.nstride, cnt, pos, [&](uint32_t, const float4& xj) {
@@ -456,11 +468,12 @@ __global__ __launch_bounds__(BS) void k_neighbors_lambda(StepConsts K, const Sce
             float r2 = rx * rx + ry * ry + rz * rz;
             A.add(rx, ry, rz, r2, K);
         });
-        float lam = A.lambda(K);
+        const float lam = A.lambda(K);
         reinterpret_cast<float*>(pos + s)[3] = lam;
         lam1[s] = lam;
         if (dbg_lam) dbg_lam[s] = lam;
         if (L.feat) {
+            const SceneGeom G = geom_of(K, geoms, sid, s);
             const float4 v = L.vel[s];
             const float ih = 1.0f / K.h, ih2 = ih * ih;
             float f[kNumFeatures];
@@ -815,6 +828,13 @@ static inline unsigned ngrid(const Tuning& T, int n) {
 
 int reduce_blocks(const Tuning& T, int n) { return (int)ngrid(T, n > 0 ? n : 1); }
 
+// The search runs kSub lanes per particle: blocks of T.block / kSub particles.
+static inline unsigned ngrid_search(const Tuning& T, int n) {
+    const unsigned per = (unsigned)(T.block / kSub);
+    const unsigned nb = ((unsigned)n + per - 1) / per;
+    return T.grid > 0 ? std::min<unsigned>((unsigned)T.grid, nb) : nb;
+}
+
 void launch_predict_hash(hipStream_t st, const StepConsts& K, const Geometry& GE, const float4* pos, const float4* vel,
                          const StepBuffers& B) {
     if (K.n > 0)
@@ -851,8 +871,10 @@ void launch_sort_reorder(hipStream_t st, const StepConsts& K, const Geometry& GE
 void launch_neighbors_lambda(hipStream_t st, const StepConsts& K, const Geometry& GE, const Tuning& T, float4* pos,
                              const StepBuffers& B, float* dbg_lam, const LearnArgs& L) {
     if (K.n <= 0) return;
-    PBF_DISPATCH_BS(T, hipLaunchKernelGGL(k_neighbors_lambda<BS>, dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
-                                          GE.sid, pos, B.cstart, B.nbr, B.ncnt, B.lam1, dbg_lam, L, T.xcd_map));
+    PBF_DISPATCH_BS(T, hipLaunchKernelGGL(k_search<BS>, dim3(ngrid_search(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
+                                          GE.sid, pos, B.cstart, B.nbr, B.ncnt, T.xcd_map));
+    PBF_DISPATCH_BS(T, hipLaunchKernelGGL(k_lambda_first<BS>, dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
+                                          GE.sid, pos, B.nbr, B.ncnt, B.lam1, dbg_lam, L, T.xcd_map));
 }
 
 void launch_lambda(hipStream_t st, const StepConsts& K, const Tuning& T, float4* pos, const StepBuffers& B,
