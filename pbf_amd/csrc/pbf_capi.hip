@@ -248,7 +248,8 @@ int enqueue_step(pbf_solver* s, hipEvent_t* ev) {
     launch_scan(st, K, B);
     PROF("scan");
     if (ev) HIP_TRY(hipEventRecord(ev[2], st));
-    launch_sort_reorder(st, K, s->geo, B, B.pos[s->pc], B.ids[s->ic], B.pos[s->pc ^ 1], B.vel[s->vc ^ 1], B.ids[s->ic ^ 1],
+    launch_sort_reorder(st, K, s->geo, B, B.pos[s->pc], B.vel[s->vc], B.ids[s->ic], B.pos[s->pc ^ 1], B.vel[s->vc ^ 1],
+                        B.ids[s->ic ^ 1],
                         dbg ? s->dbg_gid : nullptr);
     PROF("scatter_rank_reorder");
     if (ev) HIP_TRY(hipEventRecord(ev[3], st));
@@ -397,7 +398,7 @@ int create_solver(const pbf_params* para, const float* grid_space, const float* 
     if ((rc = dalloc(s, &B.xold, np)) || (rc = dalloc(s, &B.xp, np)) || (rc = dalloc(s, &B.vp, np)) ||
         (rc = dalloc(s, &B.omega, np)) || (rc = dalloc(s, &B.cell, np)) || (rc = dalloc(s, &B.local, np)) ||
         (rc = dalloc(s, &B.count, mpad)) || (rc = dalloc(s, &B.cstart, mpad + 4)) ||
-        (rc = dalloc(s, &B.tile_sum, (size_t)scan_tiles(m) + 4)) || (rc = dalloc(s, &B.bucket, np)) ||
+        (rc = dalloc(s, &B.tile_sum, (size_t)scan_tiles(m) + 4)) || (rc = dalloc(s, &B.bucket, np)) || (rc = dalloc(s, &B.bucket_id, np)) ||
         (rc = dalloc(s, &B.nbr, (size_t)((s->cap + 7) / 8) * 8 * s->nstride)) || (rc = dalloc(s, &B.ncnt, np + 16)) ||
         (rc = dalloc(s, &B.lam1, np)))
         return fail(rc);
@@ -1264,7 +1265,8 @@ int pbf_step_adaptive(pbf_solver* s, int32_t use_mlp, int32_t metric, float tol,
     const bool stats = dbg || (s->P.flags & PBF_FLAG_DENSITY_STATS);
     launch_predict_hash(st, K, s->geo, B.pos[s->pc], B.vel[s->vc], B);
     launch_scan(st, K, B);
-    launch_sort_reorder(st, K, s->geo, B, B.pos[s->pc], B.ids[s->ic], B.pos[s->pc ^ 1], B.vel[s->vc ^ 1], B.ids[s->ic ^ 1],
+    launch_sort_reorder(st, K, s->geo, B, B.pos[s->pc], B.vel[s->vc], B.ids[s->ic], B.pos[s->pc ^ 1], B.vel[s->vc ^ 1],
+                        B.ids[s->ic ^ 1],
                         dbg ? s->dbg_gid : nullptr);
     int a = s->pc ^ 1;
     const int v = s->vc ^ 1;

@@ -14,7 +14,7 @@ struct StepBuffers {
     float4* vel[2];
     uint32_t* ids[2];
     float4* xold;
-    float4* xp;
+    float4* xp;          // xp / vp: staging of host transfers (free during a step)
     float4* vp;
     float4* omega;
     uint32_t* cell;
@@ -23,6 +23,7 @@ struct StepBuffers {
     uint32_t* cstart;    // ncells + 2
     uint32_t* tile_sum;  // scan tiles
     uint32_t* bucket;
+    uint32_t* bucket_id; // particle id of each bucket entry (ranking inside a cell)
     uint32_t* nbr;       // ceil(cap / 8) * 8 * nstride, blocks of 8 entries per slot
     uint8_t* ncnt;
     float* lam1;         // iteration-1 lambda per slot
@@ -62,15 +63,16 @@ int scan_tiles(int m);
 // Workgroups (= reduction partials) of a neighbour-kernel launch over n particles.
 int reduce_blocks(const Tuning& T, int n);
 
-// predict + clamp + hash + count (reads the state, writes xp / vp / cell / local / count)
+// predict + clamp + hash + count (reads the state, writes cell / local / count)
 void launch_predict_hash(hipStream_t st, const StepConsts& K, const Geometry& GE, const float4* pos, const float4* vel,
                          const StepBuffers& B);
 // exclusive scan of count into cstart; clears count for the next step
 void launch_scan(hipStream_t st, const StepConsts& K, const StepBuffers& B);
-// scatter into buckets, rank by particle id, move xp / vp / start-of-step position / id to slot order
+// scatter into buckets, rank by particle id; the predicted state (recomputed from the old one),
+// the start-of-step position and the id move to slot order
 void launch_sort_reorder(hipStream_t st, const StepConsts& K, const Geometry& GE, const StepBuffers& B,
-                         const float4* pos_in, const uint32_t* ids_in, float4* pos_out, float4* vel_out,
-                         uint32_t* ids_out, int32_t* dbg_gid);
+                         const float4* pos_in, const float4* vel_in, const uint32_t* ids_in, float4* pos_out,
+                         float4* vel_out, uint32_t* ids_out, int32_t* dbg_gid);
 void launch_neighbors_lambda(hipStream_t st, const StepConsts& K, const Geometry& GE, const Tuning& T, float4* pos,
                              const StepBuffers& B, float* dbg_lam, const LearnArgs& L);
 void launch_lambda(hipStream_t st, const StepConsts& K, const Tuning& T, float4* pos, const StepBuffers& B,

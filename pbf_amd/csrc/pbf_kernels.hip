@@ -85,19 +85,10 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
     return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
 }
 
-// ---------------------------------------------------------------------------------------
-// updateParticles (cu:4-13) + calculateParticlesGid (cu:31-40): predict, clamp, hash, count.
-// The returned atomic slot gives an (unordered) place inside the cell bucket.
-__global__ __launch_bounds__(kBlock) void k_predict_hash(StepConsts K, const SceneGeom* __restrict__ geoms,
-                                                         const uint8_t* __restrict__ sid, const float4* __restrict__ pos,
-                                                         const float4* __restrict__ vel, float4* __restrict__ xp,
-                                                         float4* __restrict__ vp, uint32_t* __restrict__ cell,
-                                                         uint32_t* __restrict__ local, uint32_t* __restrict__ count) {
-    int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= K.n) return;  // whole tail lanes leave; ballots below count the active lanes only
-    const SceneGeom G = geom_of(K, geoms, sid, s);
-    float4 x = pos[s];
-    float4 v = vel[s];
+// updateParticles (cu:4-13) + BoundParticle: v += dt g, x += dt v, clamp.  Evaluated by the hash
+// pass for the cell and again by the reorder for the state it writes (same operations, same bits),
+// so the predicted state is never stored in between.
+__device__ __forceinline__ void predict(float4& x, float4& v, const StepConsts& K, const SceneGeom& G) {
     v.x = v.x + K.dt * K.g[0];
     v.y = v.y + K.dt * K.g[1];
     v.z = v.z + K.dt * K.g[2];
@@ -105,6 +96,21 @@ __global__ __launch_bounds__(kBlock) void k_predict_hash(StepConsts K, const Sce
     x.y = x.y + K.dt * v.y;
     x.z = x.z + K.dt * v.z;
     bound_particle(x.x, x.y, x.z, K, G);
+}
+
+// ---------------------------------------------------------------------------------------
+// updateParticles (cu:4-13) + calculateParticlesGid (cu:31-40): predict, clamp, hash, count.
+// The returned atomic slot gives an (unordered) place inside the cell bucket.
+__global__ __launch_bounds__(kBlock) void k_predict_hash(StepConsts K, const SceneGeom* __restrict__ geoms,
+                                                         const uint8_t* __restrict__ sid, const float4* __restrict__ pos,
+                                                         const float4* __restrict__ vel, uint32_t* __restrict__ cell,
+                                                         uint32_t* __restrict__ local, uint32_t* __restrict__ count) {
+    int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= K.n) return;  // whole tail lanes leave; ballots below count the active lanes only
+    const SceneGeom G = geom_of(K, geoms, sid, s);
+    float4 x = pos[s];
+    float4 v = vel[s];
+    predict(x, v, K, G);
     int c = cell_of(x.x, x.y, x.z, K, G);
     // Histogram with one atomic per run of equal cells in the wavefront (slots are in the
     // previous step's cell order, so runs are long): the run head adds the run length and every
@@ -123,8 +129,6 @@ __global__ __launch_bounds__(kBlock) void k_predict_hash(StepConsts K, const Sce
     if (head) base = atomicAdd(&count[c], (uint32_t)(run_end - lane));
     base = __shfl(base, hpos, 64);
     const uint32_t l = base + (uint32_t)(lane - hpos);
-    xp[s] = make_float4(x.x, x.y, x.z, 0.0f);
-    vp[s] = v;  // .w carries the previous step's accumulated multiplier (warm start)
     cell[s] = (uint32_t)c;
     local[s] = l;
 }
@@ -223,10 +227,14 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(uint32_t* __restrict__ cou
 // countingSort (cu:112-122) scatter into cell buckets, unordered inside a bucket.
 __global__ __launch_bounds__(kBlock) void k_scatter(int n, const uint32_t* __restrict__ cell,
                                                     const uint32_t* __restrict__ local,
-                                                    const uint32_t* __restrict__ cstart, uint32_t* __restrict__ bucket) {
+                                                    const uint32_t* __restrict__ cstart,
+                                                    const uint32_t* __restrict__ ids_in, uint32_t* __restrict__ bucket,
+                                                    uint32_t* __restrict__ bucket_id) {
     int s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= n) return;
-    bucket[cstart[cell[s]] + local[s]] = (uint32_t)s;
+    const uint32_t d = cstart[cell[s]] + local[s];
+    bucket[d] = (uint32_t)s;
+    bucket_id[d] = ids_in[s];  // the cell's member ids side by side for the ranking
 }
 
 // Orders each bucket by ascending particle id (rank = members with a smaller id) and moves the
@@ -235,30 +243,33 @@ __global__ __launch_bounds__(kBlock) void k_scatter(int n, const uint32_t* __res
 __global__ __launch_bounds__(kBlock) void k_rank_reorder(StepConsts K, const SceneGeom* __restrict__ geoms,
                                                          const uint8_t* __restrict__ sid,
                                                          const uint32_t* __restrict__ bucket,
+                                                         const uint32_t* __restrict__ bucket_id,
                                                          const uint32_t* __restrict__ cell,
                                                          const uint32_t* __restrict__ cstart,
-                                                         const uint32_t* __restrict__ ids_in,
-                                                         const float4* __restrict__ xp, const float4* __restrict__ vp,
+                                                         const float4* __restrict__ vel_in,
                                                          const float4* __restrict__ pos_in, float4* __restrict__ pos_out,
                                                          float4* __restrict__ vel_out, float4* __restrict__ xold_out,
                                                          uint32_t* __restrict__ ids_out, int32_t* __restrict__ dbg_gid) {
     int t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= K.n) return;
-    uint32_t e = bucket[t];
-    uint32_t c = cell[e];
-    uint32_t id = ids_in[e];
+    const uint32_t e = bucket[t];
+    const uint32_t id = bucket_id[t];
+    const uint32_t c = cell[e];
     const SceneGeom G = geom_of(K, geoms, sid, (int)e);
     const bool inside = (int)c != G.coff + G.ncells;
     int d = t;
     if (inside) {
-        uint32_t st = cstart[c], en = cstart[c + 1];
+        const uint32_t st = cstart[c], en = cstart[c + 1];
         uint32_t rank = 0;
-        for (uint32_t q = st; q < en; ++q) rank += ids_in[bucket[q]] < id ? 1u : 0u;
+        for (uint32_t q = st; q < en; ++q) rank += bucket_id[q] < id ? 1u : 0u;
         d = (int)(st + rank);
     }
-    pos_out[d] = xp[e];
-    vel_out[d] = vp[e];
-    xold_out[d] = pos_in[e];
+    const float4 xo = pos_in[e];
+    float4 x = xo, v = vel_in[e];  // v.w: the previous step's iteration-1 lambda, carried along
+    predict(x, v, K, G);
+    pos_out[d] = make_float4(x.x, x.y, x.z, 0.0f);
+    vel_out[d] = v;
+    xold_out[d] = xo;
     ids_out[d] = id;
     if (dbg_gid) dbg_gid[id] = inside ? (int32_t)c - G.coff : -1;
 }
@@ -838,8 +849,8 @@ static inline unsigned ngrid_search(const Tuning& T, int n) {
 void launch_predict_hash(hipStream_t st, const StepConsts& K, const Geometry& GE, const float4* pos, const float4* vel,
                          const StepBuffers& B) {
     if (K.n > 0)
-        hipLaunchKernelGGL(k_predict_hash, dim3(nblk(K.n)), dim3(kBlock), 0, st, K, GE.geoms, GE.sid, pos, vel, B.xp,
-                           B.vp, B.cell, B.local, B.count);
+        hipLaunchKernelGGL(k_predict_hash, dim3(nblk(K.n)), dim3(kBlock), 0, st, K, GE.geoms, GE.sid, pos, vel, B.cell,
+                           B.local, B.count);
 }
 
 void launch_scan(hipStream_t st, const StepConsts& K, const StepBuffers& B) {
@@ -851,13 +862,14 @@ void launch_scan(hipStream_t st, const StepConsts& K, const StepBuffers& B) {
 }
 
 void launch_sort_reorder(hipStream_t st, const StepConsts& K, const Geometry& GE, const StepBuffers& B,
-                         const float4* pos_in, const uint32_t* ids_in, float4* pos_out, float4* vel_out,
-                         uint32_t* ids_out, int32_t* dbg_gid) {
+                         const float4* pos_in, const float4* vel_in, const uint32_t* ids_in, float4* pos_out,
+                         float4* vel_out, uint32_t* ids_out, int32_t* dbg_gid) {
     int n = K.n;
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_scatter, dim3(nblk(n)), dim3(kBlock), 0, st, n, B.cell, B.local, B.cstart, B.bucket);
-    hipLaunchKernelGGL(k_rank_reorder, dim3(nblk(n)), dim3(kBlock), 0, st, K, GE.geoms, GE.sid, B.bucket, B.cell,
-                       B.cstart, ids_in, B.xp, B.vp, pos_in, pos_out, vel_out, B.xold, ids_out, dbg_gid);
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(n)), dim3(kBlock), 0, st, n, B.cell, B.local, B.cstart, ids_in, B.bucket,
+                       B.bucket_id);
+    hipLaunchKernelGGL(k_rank_reorder, dim3(nblk(n)), dim3(kBlock), 0, st, K, GE.geoms, GE.sid, B.bucket, B.bucket_id,
+                       B.cell, B.cstart, vel_in, pos_in, pos_out, vel_out, B.xold, ids_out, dbg_gid);
 }
 
 #define PBF_DISPATCH_BS(T, CALL)            \
