@@ -215,31 +215,32 @@ int pbf_mlp_train(pbf_mlp* m, const float* dev_x, const float* dev_t, int64_t ns
 /* Use m for the warm start of s (NULL detaches); m must outlive its use. */
 int pbf_attach_mlp(pbf_solver* s, pbf_mlp* m);
 
-/* One plain step with `iterations` iterations that also records, for each particle whose id
- * satisfies id % stride == phase, its 16 raw features and accumulated multiplier / h^2 at row
- * base + id / stride of the device arrays dev_x [cap][16] and dev_t [cap]. */
+/* One plain step with `iterations` (>= 2) iterations that also records, for each particle whose
+ * id satisfies id % stride == phase, its 16 raw features of pass 2 and the multiplier accumulated
+ * over passes 2..iterations, divided by h^2, at row base + id / stride of the device arrays
+ * dev_x [cap][16] and dev_t [cap]. */
 int pbf_record_step(pbf_solver* s, int32_t iterations, float* dev_x, float* dev_t, int64_t base, int64_t cap,
                     int32_t stride, int32_t phase);
 
 /* One step whose constraint loop stops once the residual is <= tol after at least min_iters
- * iterations, or after max_iters.  metric 0: compression residual, the mean over particles of
- * max(C_i, 0), C_i = rho_i/restRho - 1 over the frozen lists; metric 1: the mean of |C_i|.
- * use_mlp: iteration 1 is the learned first pass: the attached network's multiplier replaces
- * lambda for one delta-p pass, which is kept only if mean |C| does not grow (otherwise the pass
- * reruns plain from the predicted state, exactly as with use_mlp = 0); plain iterations then
- * continue until the tolerance holds (the fallback that judges the guess).  Writes the passes
- * run (the learned pass counts as one) and the last residual measured (either pointer may be
- * NULL).  Launched eagerly, one residual read-back per iteration. */
+ * iterations, or after max_iters (tol < 0: exactly max_iters, no read-back).  metric 0:
+ * compression residual, the mean over particles of max(C_i, 0), C_i = rho_i/restRho - 1 over the
+ * frozen lists; metric 1: the mean of |C_i|.  use_mlp: pass 2 is the learned pass.  After the
+ * plain pass 1, the lambda pass writes the warm-start features, the attached network's multiplier
+ * replaces lambda 2 for one delta-p pass, and the result is kept only if mean |C| does not grow
+ * (otherwise the pass reruns plain, exactly as with use_mlp = 0).  Plain iterations then continue
+ * until the tolerance holds (the fallback that judges the guess).  Writes the passes run (the
+ * learned pass counts as one) and the last residual measured (either pointer may be NULL). */
 int pbf_step_adaptive(pbf_solver* s, int32_t use_mlp, int32_t metric, float tol, int32_t min_iters, int32_t max_iters,
                       int32_t* iters_used, float* resid);
 
-/* Of the last learned first pass: out[0] / out[1] = compression of the predicted state / after
- * the network's pass, out[2] = 1 if it was kept (0 if rejected), out[3] / out[4] = mean |C| of
- * the predicted state / after the network's pass.  Synchronises the solver's stream. */
+/* Of the last learned pass: out[0] / out[1] = compression after the plain pass 1 / after the
+ * network's pass, out[2] = 1 if it was kept (0 if rejected), out[3] / out[4] = mean |C| after the
+ * plain pass 1 / after the network's pass.  Synchronises the solver's stream. */
 int pbf_last_guess_info(pbf_solver* s, float out[5]);
 
-/* passes > 0 (a network attached): every following pbf_step / pbf_advance / pbf_time_steps runs
- * the learned first pass followed by passes - 1 plain iterations, with the keep / reject decision
+/* passes >= 2 (a network attached): every following pbf_step / pbf_advance / pbf_time_steps runs
+ * a plain pass, the learned pass and passes - 2 plain iterations, with the keep / reject decision
  * made on the device (no read-back, graph-captured with PBF_FLAG_GRAPH).  passes = 0: plain
  * steps with params.iterations iterations again. */
 int pbf_set_learned(pbf_solver* s, int32_t passes);

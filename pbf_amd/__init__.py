@@ -394,8 +394,8 @@ class Solver:
         return int(it[0]), float(r[0])
 
     def set_learned(self, passes: int):
-        """passes > 0: following steps run the learned first pass + passes - 1 plain iterations
-        (device-side keep / reject, graph-captured); 0: plain steps again."""
+        """passes >= 2: following steps run a plain pass, the learned pass and passes - 2 plain
+        iterations (device-side keep / reject, graph-captured); 0: plain steps again."""
         _check(lib().pbf_set_learned(self._h, int(passes)))
 
     def snapshot(self):
@@ -407,8 +407,8 @@ class Solver:
         _check(lib().pbf_snapshot(self._h, 0))
 
     def last_guess_info(self):
-        """(compression before / after the network's pass, pass kept, mean |C| before / after) of
-        the last learned first pass."""
+        """(compression after the plain pass 1 / after the network's pass, pass kept, mean |C|
+        before / after) of the last learned pass."""
         out = np.zeros(5, np.float32)
         _check(lib().pbf_last_guess_info(self._h, _fp(out)))
         return float(out[0]), float(out[1]), bool(out[2]), float(out[3]), float(out[4])

@@ -76,6 +76,7 @@ struct pbf_solver {
     // warm start: features, accumulated multiplier, residual partials, attached network
     float* feat = nullptr;
     float* lam_sum = nullptr;
+    float* lam_keep = nullptr;
     float* resid = nullptr;
     float* resid0 = nullptr;
     float* resid_host = nullptr;
@@ -199,39 +200,52 @@ int prof_mark(pbf_solver* s, const char* name) {
         if (rc_) return rc_;                \
     } while (0)
 
-// Learned first pass: the network's multiplier (from the features the neighbour pass wrote)
-// replaces iteration 1's lambda in pos[a].w and one delta-p pass runs on it.  The gate keeps the
-// result only if mean |C| did not grow; otherwise lambda 1 is restored and the pass reruns plain
-// (both fallback launches return at once when the guess was kept).  Leaves the positions after
-// the pass in pos[a ^ 1] and, with need_lambda, their lambda in .w and residual partials in resid.
+// The learned pass (pass 2 of the step).  pos[a] holds the positions after the plain pass 1 and,
+// from the feature lambda pass, lambda 2 in .w (a copy in lam_keep) and the residual partials
+// resid0.  The network's multiplier (from the features) replaces lambda 2 and one delta-p pass
+// runs on it.  The gate keeps the result only if mean |C| did not grow over the state after pass
+// 1; otherwise lambda 2 is restored and the pass reruns plain (both fallback launches return at
+// once when the guess was kept).  Leaves the positions after the pass in pos[a ^ 1] and, with
+// need_lambda, their lambda in .w and residual partials in resid.
 int enqueue_guess_pass(pbf_solver* s, const StepConsts& K, int a, int v, bool need_lambda) {
     StepBuffers& B = s->B;
     hipStream_t st = s->stream;
     const LearnArgs L0{};
     launch_mlp_infer(st, s->mlp, s->feat, s->n, B.pos[a], nullptr, s->P.h * s->P.h);
     PROF("mlp_infer");
-    launch_delta_apply(st, K, s->geo, s->tune, B.pos[a], B.pos[a ^ 1], B.vel[v], B, 0, 1, L0);
+    launch_delta_apply(st, K, s->geo, s->tune, B.pos[a], B.pos[a ^ 1], B.vel[v], B, 0, 0, L0);
     PROF("delta_apply");
     LearnArgs LR{};
     LR.resid = s->resid;
-    launch_lambda(st, K, s->tune, B.pos[a ^ 1], B, LR);
+    launch_lambda(st, K, s->geo, s->tune, B.pos[a ^ 1], B, LR);
     PROF("lambda");
     launch_guess_gate(st, s->resid0, s->resid, reduce_blocks(s->tune, s->n), s->n, s->gate, s->gate_stats);
-    launch_restore_lambda(st, s->gate, B.lam1, B.pos[a], s->n);
+    launch_restore_lambda(st, s->gate, s->lam_keep, B.pos[a], s->n);
     LearnArgs LS{};
     LS.skip = s->gate;
-    launch_delta_apply(st, K, s->geo, s->tune, B.pos[a], B.pos[a ^ 1], B.vel[v], B, 0, 1, LS);
+    launch_delta_apply(st, K, s->geo, s->tune, B.pos[a], B.pos[a ^ 1], B.vel[v], B, 0, 0, LS);
     if (need_lambda) {
         LS.resid = s->resid;
-        launch_lambda(st, K, s->tune, B.pos[a ^ 1], B, LS);
+        launch_lambda(st, K, s->geo, s->tune, B.pos[a ^ 1], B, LS);
     }
     PROF("guess_gate_fallback");
     HIP_TRY(hipGetLastError());
     return PBF_OK;
 }
 
-// Enqueue one step: plain (P.iterations Jacobi iterations) or, with learned_passes > 0, the
-// learned first pass followed by learned_passes - 1 plain iterations.  Stage events (7 stage
+// Arguments of the feature lambda pass (pass 2's lambda with the warm-start features).
+LearnArgs feature_args(pbf_solver* s, int v, float* resid) {
+    LearnArgs L{};
+    L.feat = s->feat;
+    L.vel = s->B.vel[v];
+    L.lam1 = s->B.lam1;
+    L.lam_keep = s->lam_keep;
+    L.resid = resid;
+    return L;
+}
+
+// Enqueue one step: plain (P.iterations Jacobi iterations) or, with learned_passes >= 2, a plain
+// pass, the learned pass and learned_passes - 2 plain iterations.  Stage events (7 stage
 // boundaries + start) are recorded when ev != NULL.
 int enqueue_step(pbf_solver* s, hipEvent_t* ev) {
     const StepConsts K = make_consts(s);
@@ -257,32 +271,33 @@ int enqueue_step(pbf_solver* s, hipEvent_t* ev) {
     int v = s->vc ^ 1;
     const LearnArgs L0{};
     LearnArgs LF{};
-    if (learned) {
-        LF.feat = s->feat;
-        LF.vel = B.vel[v];
-        LF.resid = s->resid0;
-    }
+    if (learned) LF.feat = s->feat;  // C at the predicted positions
     launch_neighbors_lambda(st, K, s->geo, s->tune, B.pos[a], B, dbg ? s->dbg_lam : nullptr, LF);
     PROF("neighbors_lambda");
     if (ev) HIP_TRY(hipEventRecord(ev[4], st));
     const int I = learned ? s->learned_passes : s->P.iterations;
     int it = 0;
     if (learned) {
-        int rc = enqueue_guess_pass(s, K, a, v, I > 1);
+        launch_delta_apply(st, K, s->geo, s->tune, B.pos[a], B.pos[a ^ 1], B.vel[v], B, 0, 1, L0);
+        PROF("delta_apply");
+        a ^= 1;
+        launch_lambda(st, K, s->geo, s->tune, B.pos[a], B, feature_args(s, v, s->resid0));
+        PROF("lambda_features");
+        int rc = enqueue_guess_pass(s, K, a, v, I > 2);
         if (rc) return rc;
         a ^= 1;
-        it = 1;
+        it = 2;
     }
     for (; it < I; ++it) {
-        if (it > (learned ? 1 : 0)) {
-            launch_lambda(st, K, s->tune, B.pos[a], B, L0);
+        if (it > (learned ? 2 : 0)) {
+            launch_lambda(st, K, s->geo, s->tune, B.pos[a], B, L0);
             PROF("lambda");
         }
         launch_delta_apply(st, K, s->geo, s->tune, B.pos[a], B.pos[a ^ 1], B.vel[v], B, it == I - 1, it == 0, L0);
         PROF("delta_apply");
         a ^= 1;
     }
-    if (learned && I == 1) {
+    if (learned && I == 2) {
         launch_velocity(st, K, B.pos[a], B.vel[v], B);
         PROF("velocity");
     }
@@ -403,7 +418,7 @@ int create_solver(const pbf_params* para, const float* grid_space, const float* 
         (rc = dalloc(s, &B.lam1, np)))
         return fail(rc);
     s->npartial = (int)((np + 63) / 64) + 1;  // enough for the smallest workgroup size
-    if ((rc = dalloc(s, &s->feat, np * kNumFeatures)) || (rc = dalloc(s, &s->lam_sum, np)) ||
+    if ((rc = dalloc(s, &s->feat, np * kNumFeatures)) || (rc = dalloc(s, &s->lam_sum, np)) || (rc = dalloc(s, &s->lam_keep, np)) ||
         (rc = dalloc(s, &s->resid, (size_t)s->npartial * 2)) ||
         (rc = dalloc(s, &s->resid0, (size_t)s->npartial * 2)) || (rc = dalloc(s, &s->gate, 4)) ||
         (rc = dalloc(s, &s->gate_stats, 4)))
@@ -425,6 +440,8 @@ int create_solver(const pbf_params* para, const float* grid_space, const float* 
         s->nscenes = (int)geoms->size();
     }
     launch_fill_u32(s->stream, B.count, mpad, 0u);
+    launch_fill_u32(s->stream, reinterpret_cast<uint32_t*>(s->gate), 4, 0u);
+    launch_fill_u32(s->stream, reinterpret_cast<uint32_t*>(s->gate_stats), 4, 0u);
     if (hipGetLastError() != hipSuccess) return fail(set_error(PBF_ERR_HIP, "fill launch failed"));
     *out = s;
     rc = pbf_set_state(s, pos_xyz, vel_xyz);
@@ -799,7 +816,7 @@ int pbf_time_stage(pbf_solver* s, int32_t stage, int32_t reps, float* ms) {
     HIP_TRY(hipEventRecord(e0, s->stream));
     for (int r = 0; r < reps; ++r) {
         if (stage == 0) launch_neighbors_lambda(s->stream, K, s->geo, s->tune, B.pos[a], B, nullptr, L0);
-        if (stage == 1) launch_lambda(s->stream, K, s->tune, B.pos[a], B, L0);
+        if (stage == 1) launch_lambda(s->stream, K, s->geo, s->tune, B.pos[a], B, L0);
         if (stage == 2) launch_delta_apply(s->stream, K, s->geo, s->tune, B.pos[a], B.pos[a ^ 1], B.vel[s->vc], B, 0, 1, L0);
     }
     HIP_TRY(hipEventRecord(e1, s->stream));
@@ -1224,17 +1241,105 @@ int pbf_memcpy_to_device(void* dst, const void* src, uint64_t bytes) {
     return PBF_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// One step whose constraint loop stops at residual <= tol (see pbf_step_adaptive).  record: the
+// pass-2 lambda writes the warm-start features and lam_sum accumulates the multipliers of passes
+// 2..I (the training target); the residual is not read.
+int step_adaptive(pbf_solver* s, int32_t use_mlp, int32_t metric, float tol, int32_t min_iters, int32_t max_iters,
+                  int32_t* iters_used, float* resid, bool record) {
+    if (!s || max_iters < 1 || min_iters < 0 || metric < 0 || metric > 1) return set_error(PBF_ERR_ARG, "bad argument");
+    if (use_mlp && !s->mlp) return set_error(PBF_ERR_STATE, "no network attached");
+    HIP_TRY(hipSetDevice(s->device));
+    const StepConsts K = make_consts(s);
+    StepBuffers& B = s->B;
+    hipStream_t st = s->stream;
+    const bool dbg = (s->P.flags & PBF_FLAG_DEBUG) != 0;
+    const bool stats = dbg || (s->P.flags & PBF_FLAG_DENSITY_STATS);
+    const bool features = use_mlp || record;
+    launch_predict_hash(st, K, s->geo, B.pos[s->pc], B.vel[s->vc], B);
+    launch_scan(st, K, B);
+    launch_sort_reorder(st, K, s->geo, B, B.pos[s->pc], B.vel[s->vc], B.ids[s->ic], B.pos[s->pc ^ 1], B.vel[s->vc ^ 1],
+                        B.ids[s->ic ^ 1], dbg ? s->dbg_gid : nullptr);
+    int a = s->pc ^ 1;
+    const int v = s->vc ^ 1;
+    LearnArgs LF{};
+    if (features) LF.feat = s->feat;
+    launch_neighbors_lambda(st, K, s->geo, s->tune, B.pos[a], B, dbg ? s->dbg_lam : nullptr, LF);
+    const LearnArgs L0{};
+    LearnArgs LA{};
+    if (record) LA.lam_sum = s->lam_sum;
+    LearnArgs LR{};
+    LR.resid = s->resid;
+    const bool check = tol >= 0.0f && !record;  // read residuals back
+    int it = 0;
+    float r = -1.0f;
+    bool done = false;
+    while (!done) {
+        // pass it + 1 on the lambda in pos[a].w
+        launch_delta_apply(st, K, s->geo, s->tune, B.pos[a], B.pos[a ^ 1], B.vel[v], B, 0, it == 1,
+                           it >= 1 ? LA : L0);
+        a ^= 1;
+        ++it;
+        if (it >= max_iters) break;
+        const bool feat_pass = features && it == 1;
+        launch_lambda(st, K, s->geo, s->tune, B.pos[a], B, feat_pass ? feature_args(s, v, s->resid0) : LR);
+        HIP_TRY(hipGetLastError());
+        if (check && it >= min_iters) {
+            float comp = 0.0f, dev = 0.0f;
+            int rc = read_residual(s, feat_pass ? s->resid0 : s->resid, &comp, &dev);
+            if (rc) return rc;
+            r = metric ? dev : comp;
+            if (r <= tol) break;
+        }
+        if (use_mlp && it == 1) {
+            // the learned pass (pass 2)
+            int rc = enqueue_guess_pass(s, K, a, v, max_iters > 2);
+            if (rc) return rc;
+            a ^= 1;
+            ++it;
+            if (it >= max_iters) break;
+            if (check && it >= min_iters) {
+                float comp = 0.0f, dev = 0.0f;
+                rc = read_residual(s, s->resid, &comp, &dev);
+                if (rc) return rc;
+                r = metric ? dev : comp;
+                if (r <= tol) break;
+            }
+        }
+    }
+    launch_velocity(st, K, B.pos[a], B.vel[v], B);
+    launch_curl(st, K, s->tune, B.pos[a], B.vel[v], B, dbg ? s->dbg_err : nullptr, stats ? s->partial : nullptr);
+    launch_vort_apply(st, K, s->tune, B.pos[a], B.vel[v], B.vel[v ^ 1], B);
+    HIP_TRY(hipGetLastError());
+    s->lam_last = a ^ 1;
+    s->pc = a;
+    s->vc = v ^ 1;
+    s->ic ^= 1;
+    s->stats_valid = stats;
+    s->steps_done++;
+    if (iters_used) *iters_used = it;
+    if (resid) *resid = r;
+    return PBF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 // One plain step with `iterations` Jacobi iterations that also records, for every particle
 // with id % stride == phase, its raw features (at the predicted positions) and the multiplier
 // the step accumulated over its iterations divided by h^2, at row base + id / stride of
 // (dev_x [cap][16], dev_t [cap]).
 int pbf_record_step(pbf_solver* s, int32_t iterations, float* dev_x, float* dev_t, int64_t base, int64_t cap,
                     int32_t stride, int32_t phase) {
-    if (!s || !dev_x || !dev_t || iterations < 1 || stride < 1 || phase < 0 || phase >= stride || base < 0)
+    if (!s || !dev_x || !dev_t || iterations < 2 || stride < 1 || phase < 0 || phase >= stride || base < 0)
         return set_error(PBF_ERR_ARG, "bad argument");
     HIP_TRY(hipSetDevice(s->device));
     int32_t used = 0;
-    int rc = pbf_step_adaptive(s, 0, 0, -1.0f, iterations, iterations, &used, nullptr);
+    int rc = step_adaptive(s, 0, 0, -1.0f, iterations, iterations, &used, nullptr, true);
     if (rc) return rc;
     const float inv_h2 = 1.0f / (s->P.h * s->P.h);
     launch_record(s->stream, s->n, s->B.ids[s->ic], s->feat, s->lam_sum, inv_h2, stride, phase, (size_t)base,
@@ -1255,77 +1360,10 @@ int pbf_record_step(pbf_solver* s, int32_t iterations, float* dev_x, float* dev_
 // be NULL) report the passes run (the learned pass counts as one) and the last residual measured.
 int pbf_step_adaptive(pbf_solver* s, int32_t use_mlp, int32_t metric, float tol, int32_t min_iters, int32_t max_iters,
                       int32_t* iters_used, float* resid) {
-    if (!s || max_iters < 1 || min_iters < 0 || metric < 0 || metric > 1) return set_error(PBF_ERR_ARG, "bad argument");
-    if (use_mlp && !s->mlp) return set_error(PBF_ERR_STATE, "no network attached");
-    HIP_TRY(hipSetDevice(s->device));
-    const StepConsts K = make_consts(s);
-    StepBuffers& B = s->B;
-    hipStream_t st = s->stream;
-    const bool dbg = (s->P.flags & PBF_FLAG_DEBUG) != 0;
-    const bool stats = dbg || (s->P.flags & PBF_FLAG_DENSITY_STATS);
-    launch_predict_hash(st, K, s->geo, B.pos[s->pc], B.vel[s->vc], B);
-    launch_scan(st, K, B);
-    launch_sort_reorder(st, K, s->geo, B, B.pos[s->pc], B.vel[s->vc], B.ids[s->ic], B.pos[s->pc ^ 1], B.vel[s->vc ^ 1],
-                        B.ids[s->ic ^ 1],
-                        dbg ? s->dbg_gid : nullptr);
-    int a = s->pc ^ 1;
-    const int v = s->vc ^ 1;
-    LearnArgs LF{};
-    LF.feat = s->feat;
-    LF.vel = B.vel[v];
-    LF.resid = use_mlp ? s->resid0 : nullptr;
-    launch_neighbors_lambda(st, K, s->geo, s->tune, B.pos[a], B, dbg ? s->dbg_lam : nullptr, LF);
-    LearnArgs LA{};
-    if (!use_mlp) LA.lam_sum = s->lam_sum;
-    LearnArgs LR{};
-    LR.resid = s->resid;
-    int it = 0;
-    float r = -1.0f;
-    bool done = false;
-    if (use_mlp) {
-        int rc = enqueue_guess_pass(s, K, a, v, max_iters > 1);
-        if (rc) return rc;
-        a ^= 1;
-        it = 1;
-        done = it >= max_iters;
-        if (!done && tol >= 0.0f && it >= min_iters) {
-            float comp = 0.0f, dev = 0.0f;
-            rc = read_residual(s, s->resid, &comp, &dev);
-            if (rc) return rc;
-            r = metric ? dev : comp;
-            done = r <= tol;
-        }
-    }
-    while (!done) {
-        launch_delta_apply(st, K, s->geo, s->tune, B.pos[a], B.pos[a ^ 1], B.vel[v], B, 0, it == 0, LA);
-        a ^= 1;
-        ++it;
-        if (it >= max_iters) break;
-        launch_lambda(st, K, s->tune, B.pos[a], B, LR);
-        HIP_TRY(hipGetLastError());
-        if (tol < 0.0f || it < min_iters) continue;  // nothing to test: no read-back
-        float rc_comp = 0.0f, rc_dev = 0.0f;
-        int rc = read_residual(s, s->resid, &rc_comp, &rc_dev);
-        if (rc) return rc;
-        r = metric ? rc_dev : rc_comp;
-        if (r <= tol) break;
-    }
-    launch_velocity(st, K, B.pos[a], B.vel[v], B);
-    launch_curl(st, K, s->tune, B.pos[a], B.vel[v], B, dbg ? s->dbg_err : nullptr, stats ? s->partial : nullptr);
-    launch_vort_apply(st, K, s->tune, B.pos[a], B.vel[v], B.vel[v ^ 1], B);
-    HIP_TRY(hipGetLastError());
-    s->lam_last = a ^ 1;
-    s->pc = a;
-    s->vc = v ^ 1;
-    s->ic ^= 1;
-    s->stats_valid = stats;
-    s->steps_done++;
-    if (iters_used) *iters_used = it;
-    if (resid) *resid = r;
-    return PBF_OK;
+    return step_adaptive(s, use_mlp, metric, tol, min_iters, max_iters, iters_used, resid, false);
 }
 
-// The last learned first pass: out[0] / out[1] = compression of the predicted state / after the
+// The last learned pass: out[0] / out[1] = compression after the plain pass 1 / after the
 // network's pass, out[2] = 1 if that pass was kept, out[3] / out[4] = mean |C| before / after.
 int pbf_last_guess_info(pbf_solver* s, float out[5]) {
     if (!s || !out) return set_error(PBF_ERR_ARG, "NULL argument");
@@ -1343,10 +1381,12 @@ int pbf_last_guess_info(pbf_solver* s, float out[5]) {
     return PBF_OK;
 }
 
-// passes > 0: every following pbf_step / pbf_advance / pbf_time_steps runs the learned first pass
-// (network attached) and passes - 1 plain iterations; 0: back to P.iterations plain iterations.
+// passes >= 2: every following pbf_step / pbf_advance / pbf_time_steps runs a plain pass, the
+// learned pass (network attached) and passes - 2 plain iterations; 0: back to P.iterations plain
+// iterations.
 int pbf_set_learned(pbf_solver* s, int32_t passes) {
-    if (!s || passes < 0 || passes > 1000) return set_error(PBF_ERR_ARG, "bad argument");
+    if (!s || passes < 0 || passes == 1 || passes > 1000)
+        return set_error(PBF_ERR_ARG, "passes: 0 (plain) or 2..1000 (a plain pass, the learned pass, the rest plain)");
     if (passes > 0 && !s->mlp) return set_error(PBF_ERR_STATE, "no network attached");
     s->learned_passes = passes;
     return PBF_OK;

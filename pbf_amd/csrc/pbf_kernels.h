@@ -29,18 +29,22 @@ struct StepBuffers {
     float* lam1;         // iteration-1 lambda per slot
 };
 
-// Warm-start features per particle (raw fp32, written by the neighbour pass when requested):
-//  0 C_i = rho/rho0 - 1 at the predicted position   1 neighbour count
-//  2 sum_j |grad_j C_i|^2                             3 |grad_i C_i|
-//  4-6 v * dt / h (predicted velocity)                7 previous step's iteration-1 lambda / h^2
-//  8 distance to the nearest wall / h (<= 4)          9-14 distance to each wall (x-,x+,y-,y+,z-,z+) / h
-//  15 lambda of iteration 1 / h^2
+// Warm-start features per particle (raw fp32).  The learned step runs one plain pass first; the
+// features describe the particle after it and are written by the pass-2 lambda (slot 10 by the
+// first lambda):
+//  0 C_i = rho/rho0 - 1 after pass 1              1 neighbour count
+//  2 sum_j |grad_j C_i|^2 after pass 1            3 |grad_i C_i| after pass 1
+//  4-6 v * dt / h (predicted velocity)            7 distance to the nearest wall / h (<= 4)
+//  8 lambda of pass 2 / h^2                       9 lambda of pass 1 / h^2
+//  10 C_i at the predicted positions              11-15 distance to the walls x-, x+, y-, y+, z- / h (<= 4)
 constexpr int kNumFeatures = 16;
 
 // Optional outputs of the constraint kernels (all NULL on the plain path).
 struct LearnArgs {
     float* feat;         // [n][kNumFeatures] raw features (neighbour pass)
-    const float4* vel;   // predicted velocity, previous step's iteration-1 lambda in .w (features)
+    const float4* vel;   // predicted velocity (features)
+    const float* lam1;   // iteration-1 lambda (features)
+    float* lam_keep;     // pass-2 lambda, kept for the fallback of a rejected guess
     float* lam_sum;      // accumulated multiplier of this step (delta-p passes)
     float* resid;        // per-workgroup (sum max(C, 0), sum |C|) pairs (lambda passes)
     const int* skip;     // delta-p / lambda passes return at once when *skip != 0
@@ -75,8 +79,9 @@ void launch_sort_reorder(hipStream_t st, const StepConsts& K, const Geometry& GE
                          float4* vel_out, uint32_t* ids_out, int32_t* dbg_gid);
 void launch_neighbors_lambda(hipStream_t st, const StepConsts& K, const Geometry& GE, const Tuning& T, float4* pos,
                              const StepBuffers& B, float* dbg_lam, const LearnArgs& L);
-void launch_lambda(hipStream_t st, const StepConsts& K, const Tuning& T, float4* pos, const StepBuffers& B,
-                   const LearnArgs& L);
+// lambda over the frozen list; with L.feat also the pass-2 warm-start features and L.lam_keep
+void launch_lambda(hipStream_t st, const StepConsts& K, const Geometry& GE, const Tuning& T, float4* pos,
+                   const StepBuffers& B, const LearnArgs& L);
 void launch_delta_apply(hipStream_t st, const StepConsts& K, const Geometry& GE, const Tuning& T, const float4* pos,
                         float4* pos_next, float4* vel, const StepBuffers& B, int last, int first, const LearnArgs& L);
 void launch_curl(hipStream_t st, const StepConsts& K, const Tuning& T, const float4* pos, float4* vel,
@@ -85,7 +90,7 @@ void launch_vort_apply(hipStream_t st, const StepConsts& K, const Tuning& T, con
                        float4* vel_out, const StepBuffers& B);
 // learned first pass: accept / reject on the device, and the fallback's lambda restore
 void launch_guess_gate(hipStream_t st, const float* r0, const float* rg, int np, int n, int* gate, float* stats);
-void launch_restore_lambda(hipStream_t st, const int* gate, const float* lam1, float4* pos, int n);
+void launch_restore_lambda(hipStream_t st, const int* gate, const float* lam, float4* pos, int n);
 void launch_velocity(hipStream_t st, const StepConsts& K, const float4* pos, float4* vel, const StepBuffers& B);
 void launch_export(hipStream_t st, int n, const float4* pos, const float4* vel, const uint32_t* ids, float* pos_xyz,
                    float* vel_xyz);

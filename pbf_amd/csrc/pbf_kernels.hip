@@ -455,10 +455,9 @@ __global__ __launch_bounds__(BS) void k_search(StepConsts K, const SceneGeom* __
 }
 
 // The first calculateLambda (cu:156-180), over the list just built.  Lambda of iteration 1 goes
-// to pos.w and to lam1 (kept for the learned step's fallback and, through vel.w, as the next
-// step's feature).  With L.feat the raw warm-start features of the particle are written as well
-// (kNumFeatures floats, see pbf_kernels.h); with L.resid the workgroup sums of max(C, 0) and |C|
-// at the predicted positions.
+// to pos.w and to lam1 (a warm-start feature of pass 2, and carried to the next step in vel.w).
+// With L.feat, C at the predicted positions goes to feature slot 10; with L.resid the workgroup
+// sums of max(C, 0) and |C| at the predicted positions.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_lambda_first(StepConsts K, const SceneGeom* __restrict__ geoms,
                                                      const uint8_t* __restrict__ sid, float4* __restrict__ pos,
@@ -483,37 +482,8 @@ __global__ __launch_bounds__(BS) void k_lambda_first(StepConsts K, const SceneGe
         reinterpret_cast<float*>(pos + s)[3] = lam;
         lam1[s] = lam;
         if (dbg_lam) dbg_lam[s] = lam;
-        if (L.feat) {
-            const SceneGeom G = geom_of(K, geoms, sid, s);
-            const float4 v = L.vel[s];
-            const float ih = 1.0f / K.h, ih2 = ih * ih;
-            float f[kNumFeatures];
-            f[0] = A.rho / K.restRho - 1.0f;
-            f[1] = (float)cnt;
-            f[2] = A.gms / (K.restRho * K.restRho);
-            f[3] = sqrtf(A.gx * A.gx + A.gy * A.gy + A.gz * A.gz) / K.restRho;
-            f[4] = v.x * K.dt * ih;
-            f[5] = v.y * K.dt * ih;
-            f[6] = v.z * K.dt * ih;
-            f[7] = v.w * ih2;
-            float dmin = 4.0f;
-            const float p3[3] = {xi.x, xi.y, xi.z};
-            for (int a = 0; a < 3; ++a) {
-                float lo = fminf(fmaxf((p3[a] - G.bmin[a]) * ih, 0.0f), 4.0f);
-                float hi = fminf(fmaxf((G.bmax[a] - p3[a]) * ih, 0.0f), 4.0f);
-                f[9 + 2 * a] = lo;
-                f[10 + 2 * a] = hi;
-                dmin = fminf(dmin, fminf(lo, hi));
-            }
-            f[8] = dmin;
-            f[15] = lam * ih2;
-            float4* o = reinterpret_cast<float4*>(L.feat + (size_t)s * kNumFeatures);
-            o[0] = make_float4(f[0], f[1], f[2], f[3]);
-            o[1] = make_float4(f[4], f[5], f[6], f[7]);
-            o[2] = make_float4(f[8], f[9], f[10], f[11]);
-            o[3] = make_float4(f[12], f[13], f[14], f[15]);
-        }
         const float C = A.rho / K.restRho - 1.0f;
+        if (L.feat) L.feat[(size_t)s * kNumFeatures + 10] = C;  // the rest comes from the pass-2 lambda
         comp = comp + fmaxf(C, 0.0f);
         dev = dev + fabsf(C);
     }
@@ -522,9 +492,11 @@ __global__ __launch_bounds__(BS) void k_lambda_first(StepConsts K, const SceneGe
 
 // calculateLambda (cu:156-180) over the frozen list, iterations 2..I.  With L.resid, the
 // workgroup sums of max(C_i, 0) and |C_i| at the current positions are written as well.
-template <int BS>
-__global__ __launch_bounds__(BS) void k_lambda(StepConsts K, float4* __restrict__ pos, const uint32_t* __restrict__ nbr,
-                                               const uint8_t* __restrict__ ncnt, LearnArgs L, int xcd_map) {
+template <int BS, bool FEAT>
+__global__ __launch_bounds__(BS) void k_lambda(StepConsts K, const SceneGeom* __restrict__ geoms,
+                                               const uint8_t* __restrict__ sid, float4* __restrict__ pos,
+                                               const uint32_t* __restrict__ nbr, const uint8_t* __restrict__ ncnt,
+                                               LearnArgs L, int xcd_map) {
     if (L.skip && *L.skip) return;
     const BlockRange R((K.n + BS - 1) / BS, xcd_map);
     float comp = 0.0f, dev = 0.0f;
@@ -540,8 +512,41 @@ __global__ __launch_bounds__(BS) void k_lambda(StepConsts K, float4* __restrict_
             float r2 = rx * rx + ry * ry + rz * rz;
             A.add(rx, ry, rz, r2, K);
         });
-        reinterpret_cast<float*>(pos + s)[3] = A.lambda(K);
+        const float lam = A.lambda(K);
+        reinterpret_cast<float*>(pos + s)[3] = lam;
         const float C = A.rho / K.restRho - 1.0f;
+        if constexpr (FEAT) {
+            // the pass-2 lambda: warm-start features (layout in pbf_kernels.h; slot 10 was written
+            // by the first lambda) and a copy of lambda for the fallback of a rejected guess
+            const SceneGeom G = geom_of(K, geoms, sid, s);
+            const float4 v = L.vel[s];
+            const float ih = 1.0f / K.h, ih2 = ih * ih;
+            float* f = L.feat + (size_t)s * kNumFeatures;
+            float w[6];
+            const float p3[3] = {xi.x, xi.y, xi.z};
+            float dmin = 4.0f;
+            for (int a = 0; a < 3; ++a) {
+                w[2 * a] = fminf(fmaxf((p3[a] - G.bmin[a]) * ih, 0.0f), 4.0f);
+                w[2 * a + 1] = fminf(fmaxf((G.bmax[a] - p3[a]) * ih, 0.0f), 4.0f);
+                dmin = fminf(dmin, fminf(w[2 * a], w[2 * a + 1]));
+            }
+            f[0] = C;
+            f[1] = (float)cnt;
+            f[2] = A.gms / (K.restRho * K.restRho);
+            f[3] = sqrtf(A.gx * A.gx + A.gy * A.gy + A.gz * A.gz) / K.restRho;
+            f[4] = v.x * K.dt * ih;
+            f[5] = v.y * K.dt * ih;
+            f[6] = v.z * K.dt * ih;
+            f[7] = dmin;
+            f[8] = lam * ih2;
+            f[9] = L.lam1[s] * ih2;
+            f[11] = w[0];
+            f[12] = w[1];
+            f[13] = w[2];
+            f[14] = w[3];
+            f[15] = w[4];
+            L.lam_keep[s] = lam;
+        }
         comp = comp + fmaxf(C, 0.0f);
         dev = dev + fabsf(C);
     }
@@ -756,20 +761,20 @@ __global__ __launch_bounds__(256) void k_guess_gate(const float* __restrict__ r0
     }
 }
 
-// Rejected guess: put iteration 1's lambda back into pos.w (no-op when the guess was kept).
-__global__ __launch_bounds__(kBlock) void k_restore_lambda(const int* __restrict__ gate, const float* __restrict__ lam1,
+// Rejected guess: put the saved lambda back into pos.w (no-op when the guess was kept).
+__global__ __launch_bounds__(kBlock) void k_restore_lambda(const int* __restrict__ gate, const float* __restrict__ lam,
                                                            float4* __restrict__ pos, int n) {
     if (*gate) return;
     const int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s < n) reinterpret_cast<float*>(pos + s)[3] = lam1[s];
+    if (s < n) reinterpret_cast<float*>(pos + s)[3] = lam[s];
 }
 
 void launch_guess_gate(hipStream_t st, const float* r0, const float* rg, int np, int n, int* gate, float* stats) {
     hipLaunchKernelGGL(k_guess_gate, dim3(1), dim3(256), 0, st, r0, rg, np, n > 0 ? 1.0f / (float)n : 0.0f, gate, stats);
 }
 
-void launch_restore_lambda(hipStream_t st, const int* gate, const float* lam1, float4* pos, int n) {
-    if (n > 0) hipLaunchKernelGGL(k_restore_lambda, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, gate, lam1, pos, n);
+void launch_restore_lambda(hipStream_t st, const int* gate, const float* lam, float4* pos, int n) {
+    if (n > 0) hipLaunchKernelGGL(k_restore_lambda, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, gate, lam, pos, n);
 }
 
 // updatePosition's velocity (cu:21) for a constraint loop that stopped early.
@@ -889,11 +894,16 @@ void launch_neighbors_lambda(hipStream_t st, const StepConsts& K, const Geometry
                                           GE.sid, pos, B.nbr, B.ncnt, B.lam1, dbg_lam, L, T.xcd_map));
 }
 
-void launch_lambda(hipStream_t st, const StepConsts& K, const Tuning& T, float4* pos, const StepBuffers& B,
-                   const LearnArgs& L) {
+void launch_lambda(hipStream_t st, const StepConsts& K, const Geometry& GE, const Tuning& T, float4* pos,
+                   const StepBuffers& B, const LearnArgs& L) {
     if (K.n <= 0) return;
-    PBF_DISPATCH_BS(T, hipLaunchKernelGGL(k_lambda<BS>, dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, B.nbr, B.ncnt, L,
-                                          T.xcd_map));
+    if (L.feat) {
+        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_lambda<BS, true>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
+                                              GE.sid, pos, B.nbr, B.ncnt, L, T.xcd_map));
+    } else {
+        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_lambda<BS, false>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
+                                              GE.sid, pos, B.nbr, B.ncnt, L, T.xcd_map));
+    }
 }
 
 void launch_delta_apply(hipStream_t st, const StepConsts& K, const Geometry& GE, const Tuning& T, const float4* pos,
