@@ -38,7 +38,7 @@ TARGETS = (8, 16, 32)           # plain iteration counts whose error the learned
 
 
 def record_dataset(seeds, steps, n_side=(64, 64, 64), box=(0, 0, 0, 6.4, 4.0, 3.2), record_iters=TARGET_ITERS,
-                   stride=64, log=print):
+                   stride=64, skip=0, log=print):
     per_step = None
     ds = None
     rows = 0
@@ -46,6 +46,8 @@ def record_dataset(seeds, steps, n_side=(64, 64, 64), box=(0, 0, 0, 6.4, 4.0, 3.
     for seed in seeds:
         sc = scenes.dam_break(n_side, box, seed=seed)
         s = Solver.from_scene(sc)
+        if skip > 0:
+            s.advance(skip)  # the release of the lattice: not recorded
         if per_step is None:
             per_step = -(-sc.n // stride)
             ds = DeviceDataset(per_step * steps * len(seeds))
