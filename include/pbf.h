@@ -251,9 +251,8 @@ int pbf_set_learned(pbf_solver* s, int32_t passes);
  * unchanged (the replays write only what the next step rebuilds). */
 int pbf_time_stage(pbf_solver* s, int32_t stage, int32_t reps, float* ms);
 
-/* save != 0: copy the current state (positions, velocities, particle order, and the per-particle
- * warm-start value carried with the velocity) into one device-side snapshot of the solver;
- * save == 0: restore the state from it.  Stream-ordered, no host copies. */
+/* save != 0: copy the current state (positions, velocities, particle order) into one device-side
+ * snapshot of the solver; save == 0: restore the state from it.  Stream-ordered, no host copies. */
 int pbf_snapshot(pbf_solver* s, int32_t save);
 
 /* Device memory helpers for callers without an allocator (datasets, export targets). */

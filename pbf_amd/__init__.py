@@ -399,7 +399,7 @@ class Solver:
         _check(lib().pbf_set_learned(self._h, int(passes)))
 
     def snapshot(self):
-        """Save the state (with the per-particle warm-start value) in a device-side snapshot."""
+        """Save the state in a device-side snapshot."""
         _check(lib().pbf_snapshot(self._h, 1))
 
     def restore(self):

@@ -26,7 +26,7 @@ struct StepBuffers {
     uint32_t* bucket_id; // particle id of each bucket entry (ranking inside a cell)
     uint32_t* nbr;       // ceil(cap / 8) * 8 * nstride, blocks of 8 entries per slot
     uint8_t* ncnt;
-    float* lam1;         // iteration-1 lambda per slot
+    float* lam1;         // iteration-1 lambda per slot (a warm-start feature)
 };
 
 // Warm-start features per particle (raw fp32).  The learned step runs one plain pass first; the

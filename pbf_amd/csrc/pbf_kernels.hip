@@ -265,7 +265,7 @@ __global__ __launch_bounds__(kBlock) void k_rank_reorder(StepConsts K, const Sce
         d = (int)(st + rank);
     }
     const float4 xo = pos_in[e];
-    float4 x = xo, v = vel_in[e];  // v.w: the previous step's iteration-1 lambda, carried along
+    float4 x = xo, v = vel_in[e];
     predict(x, v, K, G);
     pos_out[d] = make_float4(x.x, x.y, x.z, 0.0f);
     vel_out[d] = v;
@@ -455,7 +455,7 @@ __global__ __launch_bounds__(BS) void k_search(StepConsts K, const SceneGeom* __
 }
 
 // The first calculateLambda (cu:156-180), over the list just built.  Lambda of iteration 1 goes
-// to pos.w and to lam1 (a warm-start feature of pass 2, and carried to the next step in vel.w).
+// to pos.w and to lam1 (a warm-start feature of pass 2).
 // With L.feat, C at the predicted positions goes to feature slot 10; with L.resid the workgroup
 // sums of max(C, 0) and |C| at the predicted positions.
 template <int BS>
@@ -669,14 +669,12 @@ __global__ __launch_bounds__(BS) void k_curl(StepConsts K, const float4* __restr
     }
 }
 
-// calculateVorticityAndViscosity (cu:217-239) + applyVorticityAndViscosity (cu:24-29).  vel.w
-// leaves with the step's iteration-1 lambda (the next step's warm-start feature).
+// calculateVorticityAndViscosity (cu:217-239) + applyVorticityAndViscosity (cu:24-29).
 template <int BS>
 __global__ __launch_bounds__(BS) void k_vort_apply(StepConsts K, const float4* __restrict__ pos,
                                                    const float4* __restrict__ vel, const float4* __restrict__ omega,
                                                    const uint32_t* __restrict__ nbr, const uint8_t* __restrict__ ncnt,
-                                                   float4* __restrict__ vel_out, const float* __restrict__ lam1,
-                                                   int xcd_map) {
+                                                   float4* __restrict__ vel_out, int xcd_map) {
     const BlockRange R((K.n + BS - 1) / BS, xcd_map);
     for (int blk = R.first; blk < R.last; ++blk) {
         const int s = blk * BS + threadIdx.x;
@@ -725,7 +723,7 @@ __global__ __launch_bounds__(BS) void k_vort_apply(StepConsts K, const float4* _
         float fx = K.vort_eps * cx, fy = K.vort_eps * cy, fz = K.vort_eps * cz;
         float vx = K.visc_c * px, vy = K.visc_c * py, vz = K.visc_c * pz;
         vel_out[s] = make_float4(vi.x + (K.dt * fx + vx), vi.y + (K.dt * fy + vy), vi.z + (K.dt * fz + vz),
-                                 lam1[s]);
+                                 0.0f);
     }
 }
 
@@ -931,7 +929,7 @@ void launch_vort_apply(hipStream_t st, const StepConsts& K, const Tuning& T, con
                        float4* vel_out, const StepBuffers& B) {
     if (K.n <= 0) return;
     PBF_DISPATCH_BS(T, hipLaunchKernelGGL(k_vort_apply<BS>, dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, vel, B.omega,
-                                          B.nbr, B.ncnt, vel_out, B.lam1, T.xcd_map));
+                                          B.nbr, B.ncnt, vel_out, T.xcd_map));
 }
 
 void launch_velocity(hipStream_t st, const StepConsts& K, const float4* pos, float4* vel, const StepBuffers& B) {

@@ -92,3 +92,14 @@ def test_statefile_roundtrip(tmp_path):
     f.write_bytes(b"NOTSTATE" + raw[8:])
     with pytest.raises(ValueError):
         statefile.load(f)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys as _sys
+    import os as _os
+
+    root = _os.path.dirname(_os.path.dirname(_os.path.abspath(__file__)))
+    r = subprocess.run([_sys.executable, _os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
