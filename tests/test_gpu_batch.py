@@ -75,3 +75,29 @@ def test_autotune_keeps_state_and_picks_a_shape():
     pa, va = a.get_state()
     pb, vb = b.get_state()
     assert same_bits(pa, pb) and same_bits(va, vb)
+
+
+def test_batch_learned_step_with_rejected_network_is_plain():
+    """Learned steps on a batch (features use each scene's own walls): with a network whose pass
+    scatters the fluid, the device gate rejects it and the batch evolves exactly as with plain
+    steps of the same iteration count."""
+    import numpy as _np
+
+    from pbf_amd import Mlp
+
+    sl = small_batch()
+    print(f"[case] batched learned step, {len(sl)} scenes", flush=True)
+    plain = Solver.batch(sl, flags=FLAG_GRAPH, iterations=3)
+    lrn = Solver.batch(sl, flags=FLAG_GRAPH, iterations=3)
+    m = Mlp(seed=13)
+    rng = _np.random.default_rng(13)
+    m.set_params((rng.standard_normal(9473) * 3.0).astype(_np.float32))
+    m.set_normalization(_np.zeros(16), _np.ones(16), 1e3, 0.0)
+    lrn.attach_mlp(m)
+    lrn.set_learned(3)
+    plain.advance(5)
+    lrn.advance(5)
+    assert not lrn.last_guess_info()[2]
+    pa, va = plain.get_state()
+    pb, vb = lrn.get_state()
+    assert same_bits(pa, pb) and same_bits(va, vb)
