@@ -251,3 +251,55 @@ def test_1000_steps_density_statistics_262k():
     op, ov = o.get_state()
     assert_bitexact(gp, op, "positions after 1000 steps")
     assert_bitexact(gv, ov, "velocities after 1000 steps")
+
+
+def test_gpu_against_reference_golden_vectors():
+    """The GPU step against vectors recorded from the reference's own CPU solver on the same
+    input (tests/golden/ref_probe_3k.npz: 3375 particles, zero velocity and gravity, one
+    iteration): cell ids, cell starts / ends, per-cell member sets, neighbour sets exactly; lambda
+    of the iteration within 2e-5 and delta-p within 5e-5 of their largest magnitude.  The
+    reference's CPU scatter orders a cell's members by descending id, this solver by ascending
+    id, so members and neighbours are compared as sets."""
+    import os
+
+    from pbf_amd import SolverParameter
+
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_probe_3k.npz"))
+    pos, box, prm = g["pos"], g["box"], g["params"]
+    p = scenes.default_params(
+        h=float(prm[0]), restRho=float(prm[1]), epsilon=float(prm[2]), k=float(prm[3]), n=float(prm[4]),
+        deltaq=float(prm[5]), radius=float(prm[6]), gravity=(0.0, 0.0, 0.0), iterations=1, grad_rmin=0.01,
+        c=0.0, vorticityConfinementEpilon=0.0,
+    )
+    para = SolverParameter.from_dict(p, len(pos), (64, 64, 64), FLAG_DEBUG)
+    bd = np.concatenate([box[:3] + np.float32(prm[6]), box[3:] - np.float32(prm[6])]).astype(np.float32)
+    print(f"[case] reference probe, {len(pos)} particles", flush=True)
+    s = Solver(para, box, bd, pos, np.zeros_like(pos))
+    s.step()
+    gid, cnt, cs, srt, nin = s.grid()
+    assert nin == int(g["n_in_grid"])
+    assert np.array_equal(gid, g["gid"])
+    cells = g["cells"]
+    assert np.array_equal(cs[cells], g["cell_start"])
+    assert np.array_equal(cs[cells + 1], g["cell_end"])
+    for a, b in zip(g["cell_start"], g["cell_end"]):
+        assert list(srt[a:b]) == sorted(g["sorted_ids"][a:b])
+    nbr, ncnt = s.neighbors()
+    rn = g["nbr"]
+    rc = rn[:, 63]
+    assert np.array_equal(ncnt, rc)
+    for i in range(len(ncnt)):
+        assert set(nbr[i, : ncnt[i]]) == set(rn[i, : rc[i]])
+    lam = s.fields()[0]
+    scale = np.abs(g["lam"]).max()
+    np.testing.assert_allclose(lam, g["lam"], rtol=0, atol=2e-5 * scale)
+    # delta-p: the position change of particles the clamp did not touch, less the rounding of the
+    # addition (one ulp of the position)
+    p1, _ = s.get_state()
+    dx = p1.astype(np.float64) - pos.astype(np.float64)
+    inside = ((p1 > bd[:3]) & (p1 < bd[3:])).all(1)
+    ulp = np.spacing(np.abs(pos).astype(np.float32)).astype(np.float64)
+    scale = np.abs(g["offset"]).max()
+    err = np.abs(dx - g["offset"].astype(np.float64)) - ulp
+    assert inside.mean() > 0.5
+    assert (err[inside] <= 5e-5 * scale).all()
