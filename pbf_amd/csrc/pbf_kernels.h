@@ -64,6 +64,9 @@ struct Tuning {
 };
 
 int scan_tiles(int m);
+// Grids of at most kScanFusedMax tiles (1024 counters each) scan in two launches (tile totals,
+// then a down-sweep that adds up the totals before its tile); larger grids in three.
+constexpr int kScanFusedMax = 2048;
 // Workgroups (= reduction partials) of a neighbour-kernel launch over n particles.
 int reduce_blocks(const Tuning& T, int n);
 

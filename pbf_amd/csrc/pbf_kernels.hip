@@ -101,6 +101,8 @@ __device__ __forceinline__ void predict(float4& x, float4& v, const StepConsts& 
 // ---------------------------------------------------------------------------------------
 // updateParticles (cu:4-13) + calculateParticlesGid (cu:31-40): predict, clamp, hash, count.
 // The returned atomic slot gives an (unordered) place inside the cell bucket.
+constexpr int kScanTile = 1024;  // counters per scan tile (256 threads x 4)
+
 __global__ __launch_bounds__(kBlock) void k_predict_hash(StepConsts K, const SceneGeom* __restrict__ geoms,
                                                          const uint8_t* __restrict__ sid, const float4* __restrict__ pos,
                                                          const float4* __restrict__ vel, uint32_t* __restrict__ cell,
@@ -138,7 +140,6 @@ __global__ __launch_bounds__(kBlock) void k_predict_hash(StepConsts K, const Sce
 // addBlockSum + setGrid, cu:42-111).  Reduce-then-scan over tiles of 1024 counters: tile sums,
 // one workgroup scanning the tile sums, then every tile rescanned with its offset.  The last
 // pass also clears the counters for the next step.
-constexpr int kScanTile = 1024;
 
 __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ count, int m,
                                                         uint32_t* __restrict__ tile_sum) {
@@ -221,6 +222,59 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(uint32_t* __restrict__ cou
                 count[base + j] = 0u;
             }
     }
+}
+
+// The down-sweep without a separate scan of the tile totals (grids of up to kScanFusedMax
+// tiles): each workgroup adds up the totals k_scan_reduce wrote for the tiles before its own,
+// scans its tile as k_scan_down does and clears its counters; the last tile writes the grand
+// total.
+__global__ __launch_bounds__(kBlock) void k_scan_fused(uint32_t* __restrict__ count, int m,
+                                                       const uint32_t* __restrict__ tile_sum,
+                                                       uint32_t* __restrict__ cstart) {
+    __shared__ uint32_t wsum[kBlock / 64];
+    __shared__ uint32_t wpre[kBlock / 64];
+    const int wave = threadIdx.x >> 6;
+    uint32_t pre = 0;
+    for (int j = threadIdx.x; j < (int)blockIdx.x; j += kBlock) pre += tile_sum[j];
+    pre = wave_sum(pre);
+    if ((threadIdx.x & 63) == 0) wpre[wave] = pre;
+    int base = blockIdx.x * kScanTile + threadIdx.x * 4;
+    uint32_t c[4];
+    bool full = base + 3 < m;
+    if (full) {
+        uint4 q = *reinterpret_cast<const uint4*>(count + base);
+        c[0] = q.x;
+        c[1] = q.y;
+        c[2] = q.z;
+        c[3] = q.w;
+    } else {
+        for (int j = 0; j < 4; ++j) c[j] = base + j < m ? count[base + j] : 0u;
+    }
+    uint32_t t = c[0] + c[1] + c[2] + c[3];
+    uint32_t inc = wave_inclusive_scan(t);
+    if ((threadIdx.x & 63) == 63) wsum[wave] = inc;
+    __syncthreads();
+    uint32_t off = wpre[0] + wpre[1] + wpre[2] + wpre[3];
+    const uint32_t tile_start = off;
+    for (int w = 0; w < wave; ++w) off += wsum[w];
+    off += inc - t;
+    uint32_t o[4];
+    o[0] = off;
+    o[1] = o[0] + c[0];
+    o[2] = o[1] + c[1];
+    o[3] = o[2] + c[2];
+    if (full) {
+        *reinterpret_cast<uint4*>(cstart + base) = make_uint4(o[0], o[1], o[2], o[3]);
+        *reinterpret_cast<uint4*>(count + base) = make_uint4(0u, 0u, 0u, 0u);
+    } else {
+        for (int j = 0; j < 4; ++j)
+            if (base + j < m) {
+                cstart[base + j] = o[j];
+                count[base + j] = 0u;
+            }
+    }
+    if (threadIdx.x == 0 && (int)blockIdx.x == (int)gridDim.x - 1)
+        cstart[m] = tile_start + wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
 // ---------------------------------------------------------------------------------------
@@ -859,6 +913,11 @@ void launch_predict_hash(hipStream_t st, const StepConsts& K, const Geometry& GE
 void launch_scan(hipStream_t st, const StepConsts& K, const StepBuffers& B) {
     int m = K.nbuckets;
     int nt = scan_tiles(m);
+    if (nt <= kScanFusedMax) {
+        hipLaunchKernelGGL(k_scan_reduce, dim3(nt), dim3(kBlock), 0, st, B.count, m, B.tile_sum);
+        hipLaunchKernelGGL(k_scan_fused, dim3(nt), dim3(kBlock), 0, st, B.count, m, B.tile_sum, B.cstart);
+        return;
+    }
     hipLaunchKernelGGL(k_scan_reduce, dim3(nt), dim3(kBlock), 0, st, B.count, m, B.tile_sum);
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, st, B.tile_sum, nt, B.cstart, m);
     hipLaunchKernelGGL(k_scan_down, dim3(nt), dim3(kBlock), 0, st, B.count, m, B.tile_sum, B.cstart);
