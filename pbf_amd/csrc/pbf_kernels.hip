@@ -649,9 +649,9 @@ __global__ __launch_bounds__(BS) void k_delta_apply(StepConsts K, const SceneGeo
     }
 }
 
-// calculateCurl (cu:201-215), plus the density error rho/rho0 - 1 at the final positions when
-// err or partial is given (partial: per workgroup sum |e|, sum max(e,0), max e).
-template <int BS>
+// calculateCurl (cu:201-215), plus, with STATS, the density error rho/rho0 - 1 at the final
+// positions into err and / or partial (per workgroup sum |e|, sum max(e,0), max e).
+template <int BS, bool STATS>
 __global__ __launch_bounds__(BS) void k_curl(StepConsts K, const float4* __restrict__ pos, float4* __restrict__ vel,
                                              const uint32_t* __restrict__ nbr, const uint8_t* __restrict__ ncnt,
                                              float4* __restrict__ omega, float* __restrict__ err,
@@ -686,7 +686,7 @@ __global__ __launch_bounds__(BS) void k_curl(StepConsts K, const float4* __restr
                 wx = wx + (uy * gz - uz * gy);
                 wy = wy + (uz * gx - ux * gz);
                 wz = wz + (ux * gy - uy * gx);
-                rho = rho + poly6(r2, K);
+                if constexpr (STATS) rho = rho + poly6(r2, K);
             }
         }
         const float wlen = sqrtf(wx * wx + wy * wy + wz * wz);
@@ -694,13 +694,15 @@ __global__ __launch_bounds__(BS) void k_curl(StepConsts K, const float4* __restr
         // |omega| rides in vel.w for the vorticity pass (its gathers then need pos and vel only);
         // this pass reads only .xyz of vel.
         reinterpret_cast<float*>(vel + s)[3] = wlen;
-        float e = rho / K.restRho - 1.0f;
-        if (err) err[s] = e;
-        sa = sa + fabsf(e);
-        sp = sp + fmaxf(e, 0.0f);
-        sm = fmaxf(sm, e);
+        if constexpr (STATS) {
+            float e = rho / K.restRho - 1.0f;
+            if (err) err[s] = e;
+            sa = sa + fabsf(e);
+            sp = sp + fmaxf(e, 0.0f);
+            sm = fmaxf(sm, e);
+        }
     }
-    if (partial) {
+    if (STATS && partial) {
         __shared__ float ra[BS], rp[BS], rm[BS];
         int t = threadIdx.x;
         ra[t] = sa;
@@ -980,8 +982,13 @@ void launch_delta_apply(hipStream_t st, const StepConsts& K, const Geometry& GE,
 void launch_curl(hipStream_t st, const StepConsts& K, const Tuning& T, const float4* pos, float4* vel,
                  const StepBuffers& B, float* err, float* partial) {
     if (K.n <= 0) return;
-    PBF_DISPATCH_BS(T, hipLaunchKernelGGL(k_curl<BS>, dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, vel, B.nbr, B.ncnt,
-                                          B.omega, err, partial, T.xcd_map));
+    if (err || partial) {
+        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_curl<BS, true>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, vel,
+                                              B.nbr, B.ncnt, B.omega, err, partial, T.xcd_map));
+    } else {
+        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_curl<BS, false>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, vel,
+                                              B.nbr, B.ncnt, B.omega, err, partial, T.xcd_map));
+    }
 }
 
 void launch_vort_apply(hipStream_t st, const StepConsts& K, const Tuning& T, const float4* pos, const float4* vel,
