@@ -70,8 +70,8 @@ typedef struct pbf_params {
 typedef struct pbf_solver pbf_solver;
 
 /* Replaces initCUDASolverDataBuffer (kernels/cudaPBF.h:8-10, kernels/cudaPBF.cu:4-30):
- * allocates every device buffer for n particles and uploads positions and velocities.
- * grid_space = {pmin xyz, pmax xyz}: only pmin anchors the grid (cuh:156-160), as in the
+ * allocates every device buffer for n particles (0 <= n < 2^28) and uploads positions and
+ * velocities.  grid_space = {pmin xyz, pmax xyz}: only pmin anchors the grid (cuh:156-160), as in the
  * reference where the app passes the box (src/renderapp.cpp:251).  boundary = the clamp box
  * (box shrunk by the particle radius, renderapp.cpp:131).  vel_xyz may be NULL (zero). */
 int pbf_create(const pbf_params* para, const float grid_space[6], const float boundary[6], const float* pos_xyz,

@@ -405,10 +405,15 @@ int create_solver(const pbf_params* para, const float* grid_space, const float* 
     int m = nbuckets;
     size_t mpad = (size_t)scan_tiles(m) * 1024;
     int rc;
+    // pos / vel hold the padding slot n after the particles (see StepBuffers)
+    const float4 far_pos = make_float4(kFarPos, kFarPos, kFarPos, 0.0f), still = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     for (int i = 0; i < 2; ++i) {
-        if ((rc = dalloc(s, &B.pos[i], np))) return fail(rc);
-        if ((rc = dalloc(s, &B.vel[i], np))) return fail(rc);
+        if ((rc = dalloc(s, &B.pos[i], np + 1))) return fail(rc);
+        if ((rc = dalloc(s, &B.vel[i], np + 1))) return fail(rc);
         if ((rc = dalloc(s, &B.ids[i], np))) return fail(rc);
+        if (hipMemcpy(B.pos[i] + n, &far_pos, sizeof(float4), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(B.vel[i] + n, &still, sizeof(float4), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(set_error(PBF_ERR_HIP, "padding slot upload failed"));
     }
     if ((rc = dalloc(s, &B.xold, np)) || (rc = dalloc(s, &B.xp, np)) || (rc = dalloc(s, &B.vp, np)) ||
         (rc = dalloc(s, &B.omega, np)) || (rc = dalloc(s, &B.cell, np)) || (rc = dalloc(s, &B.local, np)) ||
@@ -463,6 +468,7 @@ int pbf_create(const pbf_params* para, const float grid_space[6], const float bo
     int rc = check_params(para);
     if (rc) return rc;
     if (n < 0) return set_error(PBF_ERR_ARG, "n < 0");
+    if (n >= kMaxSlots) return set_error(PBF_ERR_ARG, "n must be below 2^28");
     if (n > 0 && !pos_xyz) return set_error(PBF_ERR_ARG, "pos_xyz is NULL");
     if (!grid_space || !boundary) return set_error(PBF_ERR_ARG, "grid_space / boundary is NULL");
     int ncells = para->grid_dims[0] * para->grid_dims[1] * para->grid_dims[2];
@@ -497,7 +503,7 @@ int pbf_create_batch(const pbf_params* para, int32_t nscenes, const int32_t* cou
         G.ncells = (int)cells;
         nb += cells + 1;
         n += counts[k];
-        if (nb > ((int64_t)1 << 28) || n > ((int64_t)1 << 30)) return set_error(PBF_ERR_ARG, "batch too large");
+        if (nb > ((int64_t)1 << 28) || n >= kMaxSlots) return set_error(PBF_ERR_ARG, "batch too large");
     }
     if (n > 0 && !pos_xyz) return set_error(PBF_ERR_ARG, "pos_xyz is NULL");
     std::vector<uint8_t> sid((size_t)n);

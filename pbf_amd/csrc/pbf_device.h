@@ -90,11 +90,11 @@ __device__ __forceinline__ void bound_particle(float& x, float& y, float& z, con
     if (z > G.bmax[2]) z = G.bmax[2] - K.jitter * random_offset(x, y);
 }
 
-// Wpoly6, cuh:99-103, on r.r (evaluated, then selected: no branch).
+// Wpoly6, cuh:99-103, on r.r.  Outside the support h^2 - r.r < 0 is clamped to +0, which gives
+// W = +0 exactly as the reference's early return does (no compare and select).
 __device__ __forceinline__ float poly6(float r2, const StepConsts& K) {
-    float d = K.h2 - r2;
-    float w = K.c6 * (d * d * d);
-    return r2 > K.h2 ? 0.0f : w;
+    const float d = fmaxf(K.h2 - r2, 0.0f);
+    return K.c6 * (d * d * d);
 }
 
 // 1/sqrt(x): Lomont's refined bit-pattern seed 0x5f375a86 (C. Lomont, "Fast Inverse Square
@@ -109,14 +109,16 @@ __device__ __forceinline__ float pbf_rsqrt(float x) {
     return y;
 }
 
-// WspikyGrad, cuh:105-110, returned as the scalar t with grad = r * t (evaluated, then
-// selected: no branch): |r| = r.r * rsqrt(r.r) and t = cs (h - |r|)^2 * rsqrt(r.r).
+// WspikyGrad, cuh:105-110, returned as the scalar t with grad = r * t (no branch):
+// |r| = r.r * rsqrt(r.r) and t = cs (h - |r|)^2 * rsqrt(r.r).  For |r| > h, h - |r| is clamped to
+// +0, so t is +-0 instead of the reference's 0: every use multiplies t into a term of a sum that
+// starts at +0 and so can never be -0, where adding +-0 changes nothing.  |r| < rmin selects 0.
 __device__ __forceinline__ float spiky_scale(float r2, const StepConsts& K) {
     const float inv = pbf_rsqrt(r2);
     const float len = r2 * inv;
-    const float hl = K.h - len;
+    const float hl = fmaxf(K.h - len, 0.0f);
     const float t = (K.cs * (hl * hl)) * inv;
-    return (len > K.h || len < K.rmin) ? 0.0f : t;
+    return len < K.rmin ? 0.0f : t;
 }
 
 // b^e for 0 <= e < 128: the multiplications of square-and-multiply from the low bit up (the

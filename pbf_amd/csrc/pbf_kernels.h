@@ -9,6 +9,9 @@
 namespace pbf {
 
 // Device buffers of one solver; index [2] arrays alternate between steps / iterations.
+// pos[i] and vel[i] hold n + 1 entries: slot n is a padding particle at (kFarPos, kFarPos,
+// kFarPos) with zero velocity and multiplier, which no kernel writes.  The neighbour list pads
+// each particle's last group of 8 entries with n.
 struct StepBuffers {
     float4* pos[2];
     float4* vel[2];
@@ -24,7 +27,8 @@ struct StepBuffers {
     uint32_t* tile_sum;  // scan tiles
     uint32_t* bucket;
     uint32_t* bucket_id; // particle id of each bucket entry (ranking inside a cell)
-    uint32_t* nbr;       // ceil(cap / 8) * 8 * nstride, blocks of 8 entries per slot
+    uint32_t* nbr;       // ceil(cap / 8) * 8 * nstride, blocks of 8 entries per slot; entries
+                         // cnt .. ceil8(cnt) - 1 of a slot hold the padding slot n
     uint8_t* ncnt;
     float* lam1;         // iteration-1 lambda per slot (a warm-start feature)
 };
@@ -38,6 +42,16 @@ struct StepBuffers {
 //  8 lambda of pass 2 / h^2                       9 lambda of pass 1 / h^2
 //  10 C_i at the predicted positions              11-15 distance to the walls x-, x+, y-, y+, z- / h (<= 4)
 constexpr int kNumFeatures = 16;
+
+// Coordinate of the padding slot: its distance to any particle of a scene is ~1.7e18, so r.r
+// (~3e36) stays finite, poly6 and the spiky gradient are 0, and every term it adds to a
+// neighbour sum is +0 or -0.  Sums start at +0 and never become -0, so adding such a term leaves
+// them unchanged bit for bit: a padded group is evaluated as if its tail were absent.
+constexpr float kFarPos = 1e18f;
+
+// Particles per solver (all scenes of a batch) stay below 2^28, so the byte offset of any
+// float4 slot, padding slot included, fits in 32 bits (slot_ref).
+constexpr int64_t kMaxSlots = (int64_t)1 << 28;
 
 // Optional outputs of the constraint kernels (all NULL on the plain path).
 struct LearnArgs {

@@ -357,14 +357,19 @@ __device__ __forceinline__ size_t nbr_index(int s, int j, int nstride) {
 
 // Neighbour loops fetch in groups of kGroup: all list entries of the group, then all gathered
 // operands, then the group is processed in list order (the summation order of the reference's
-// loop).  Entries past cnt re-read the last valid entry and are skipped, so every fetch is in
-// bounds and branch-free.  kGroup independent loads in flight per lane instead of one.
+// loop).  kGroup independent loads in flight per lane instead of one.  The search pads the last
+// group with the padding slot n (kFarPos), whose terms are +-0 and leave every sum unchanged,
+// so a group is straight-line code with no per-entry test.
 constexpr int kGroup = 8;
 
-// Calls f(q, x_q) for the neighbours j = 0..cnt-1 of slot s in order, x_q = pos[q].
-// The 8 list entries of group j0 (a multiple of 8) of slot s; entries past cnt are replaced by
-// s itself so every gather stays in bounds.
-__device__ __forceinline__ void load_group(const uint32_t* __restrict__ nbr, int s, int j0, int cnt, int nstride,
+// Entry q of a float4 slot array by a 32-bit byte offset (q <= n < 2^28): the load takes the
+// array's scalar base and one vector offset.
+__device__ __forceinline__ const float4& slot_ref(const float4* __restrict__ base, uint32_t q) {
+    return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + (q << 4));
+}
+
+// The 8 list entries of group j0 (a multiple of 8, below cnt) of slot s.
+__device__ __forceinline__ void load_group(const uint32_t* __restrict__ nbr, int s, int j0, int nstride,
                                            uint32_t q[8]) {
     const uint4* p = reinterpret_cast<const uint4*>(nbr + nbr_index(s, j0, nstride));
     const uint4 a = p[0], b = p[1];
@@ -376,23 +381,22 @@ __device__ __forceinline__ void load_group(const uint32_t* __restrict__ nbr, int
     q[5] = b.y;
     q[6] = b.z;
     q[7] = b.w;
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-        if (j0 + u >= cnt) q[u] = (uint32_t)s;
 }
+
+// Calls f(q, x_q) for the list entries j = 0 .. ceil8(cnt) - 1 of slot s in order, x_q = pos[q]
+// (entries past cnt are the padding slot).
 
 template <class F>
 __device__ __forceinline__ void for_neighbors(const uint32_t* __restrict__ nbr, int s, int nstride, int cnt,
                                               const float4* __restrict__ pos, F&& f) {
     for (int j0 = 0; j0 < cnt; j0 += kGroup) {
         uint32_t q[kGroup];
-        load_group(nbr, s, j0, cnt, nstride, q);
+        load_group(nbr, s, j0, nstride, q);
         float4 x[kGroup];
 #pragma unroll
-        for (int u = 0; u < kGroup; ++u) x[u] = pos[q[u]];
+        for (int u = 0; u < kGroup; ++u) x[u] = slot_ref(pos, q[u]);
 #pragma unroll
-        for (int u = 0; u < kGroup; ++u)
-            if (j0 + u < cnt) f(q[u], x[u]);
+        for (int u = 0; u < kGroup; ++u) f(q[u], x[u]);
     }
 }
 
@@ -427,7 +431,9 @@ __device__ void block_sum2(float c, float d, float* __restrict__ partial) {
 // faces less a margin of 1e-3 h, which covers the rounding of the cell index.
 constexpr int kSub = 8;  // lanes per particle in the search
 
-template <int BS>
+// WIDE: the table's group stride nstride * 8 needs more than 24 bits (n > 2^21), so list
+// offsets take a full 32-bit multiply instead of the 24-bit one.
+template <int BS, bool WIDE>
 __global__ __launch_bounds__(BS) void k_search(StepConsts K, const SceneGeom* __restrict__ geoms,
                                                const uint8_t* __restrict__ sid, const float4* __restrict__ pos,
                                                const uint32_t* __restrict__ cstart, uint32_t* __restrict__ nbr,
@@ -482,8 +488,12 @@ __global__ __launch_bounds__(BS) void k_search(StepConsts K, const SceneGeom* __
             rb[r] = use ? lo : 0u;
             re[r] = use ? hi : 0u;
         }
-        // Four chunks of 8 candidates per pass: their loads are issued together.
+        // Four chunks of 8 candidates per pass: their loads are issued together.  A chunk that
+        // holds no candidate for any particle of the wavefront is skipped (a wave-uniform
+        // branch), so a run costs ceil(len / 8) chunks, not a multiple of four.
         constexpr int kChunks = 4;
+        uint32_t* const lbase = nbr + (size_t)s * 8;  // entry j at lbase[(j / 8) * gstride + j % 8]
+        const uint32_t gstride = (uint32_t)K.nstride * 8u;
         int cnt = 0;
 #pragma unroll
         for (int r = 0; r < 9; ++r) {
@@ -491,20 +501,31 @@ __global__ __launch_bounds__(BS) void k_search(StepConsts K, const SceneGeom* __
             for (uint32_t q0 = rb[r]; q0 < e && cnt < K.cap; q0 += kChunks * kSub) {
                 float4 xs[kChunks];
 #pragma unroll
-                for (int u = 0; u < kChunks; ++u) xs[u] = pos[min(q0 + (uint32_t)(u * kSub + lane8), e - 1u)];
+                for (int u = 0; u < kChunks; ++u) xs[u] = slot_ref(pos, min(q0 + (uint32_t)(u * kSub + lane8), e - 1u));
 #pragma unroll
                 for (int u = 0; u < kChunks; ++u) {
+                    if (u > 0 && __builtin_amdgcn_ballot_w64(q0 + (uint32_t)(u * kSub) < e) == 0) break;
                     const uint32_t q = q0 + (uint32_t)(u * kSub + lane8);
                     const float rx = xi.x - xs[u].x, ry = xi.y - xs[u].y, rz = xi.z - xs[u].z;
                     const bool hit = q < e && (int)q != s && rx * rx + ry * ry + rz * rz <= K.h2;
-                    const uint32_t bits = (uint32_t)(__ballot(hit) >> (kSub * wsub)) & 0xffu;
+                    const uint32_t bits = (uint32_t)(__builtin_amdgcn_ballot_w64(hit) >> (kSub * wsub)) & 0xffu;
                     const int at = cnt + __popc(bits & below);
-                    if (hit && at < K.cap) nbr[nbr_index(s, at, K.nstride)] = q;
+                    if (hit && at < K.cap) {
+                        const uint32_t g = (uint32_t)(at >> 3);
+                        lbase[(WIDE ? g * gstride : __umul24(g, gstride)) + (uint32_t)(at & 7)] = q;
+                    }
                     cnt += __popc(bits);
                 }
             }
         }
-        if (lane8 == 0) ncnt[s] = (uint8_t)min(cnt, K.cap);
+        // pad the last group of 8 with the padding slot n
+        const int c = min(cnt, K.cap);
+        const int pe = c + lane8;
+        if (pe < ((c + 7) & ~7)) {
+            const uint32_t g = (uint32_t)(pe >> 3);
+            lbase[(WIDE ? g * gstride : __umul24(g, gstride)) + (uint32_t)(pe & 7)] = (uint32_t)K.n;
+        }
+        if (lane8 == 0) ncnt[s] = (uint8_t)c;
     }
 }
 
@@ -668,15 +689,14 @@ __global__ __launch_bounds__(BS) void k_curl(StepConsts K, const float4* __restr
         for (int j0 = 0; j0 < cnt; j0 += kGroup) {
             uint32_t qs[kGroup];
             float4 xs[kGroup], vs[kGroup];
-            load_group(nbr, s, j0, cnt, K.nstride, qs);
+            load_group(nbr, s, j0, K.nstride, qs);
 #pragma unroll
             for (int u = 0; u < kGroup; ++u) {
-                xs[u] = pos[qs[u]];
-                vs[u] = vel[qs[u]];
+                xs[u] = slot_ref(pos, qs[u]);
+                vs[u] = slot_ref(vel, qs[u]);
             }
 #pragma unroll
             for (int u = 0; u < kGroup; ++u) {
-                if (j0 + u >= cnt) continue;
                 const float4 xj = xs[u], vj = vs[u];
                 float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
                 float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
@@ -743,15 +763,14 @@ __global__ __launch_bounds__(BS) void k_vort_apply(StepConsts K, const float4* _
         for (int j0 = 0; j0 < cnt; j0 += kGroup) {
             uint32_t qs[kGroup];
             float4 xs[kGroup], vs[kGroup];
-            load_group(nbr, s, j0, cnt, K.nstride, qs);
+            load_group(nbr, s, j0, K.nstride, qs);
 #pragma unroll
             for (int u = 0; u < kGroup; ++u) {
-                xs[u] = pos[qs[u]];
-                vs[u] = vel[qs[u]];
+                xs[u] = slot_ref(pos, qs[u]);
+                vs[u] = slot_ref(vel, qs[u]);
             }
 #pragma unroll
             for (int u = 0; u < kGroup; ++u) {
-                if (j0 + u >= cnt) continue;
                 const float4 xj = xs[u], vj = vs[u];
                 const float wl = vj.w;  // |omega_j|, left in vel.w by the curl pass
                 float ux = vi.x - vj.x, uy = vi.y - vj.y, uz = vi.z - vj.z;
@@ -947,8 +966,13 @@ void launch_sort_reorder(hipStream_t st, const StepConsts& K, const Geometry& GE
 void launch_neighbors_lambda(hipStream_t st, const StepConsts& K, const Geometry& GE, const Tuning& T, float4* pos,
                              const StepBuffers& B, float* dbg_lam, const LearnArgs& L) {
     if (K.n <= 0) return;
-    PBF_DISPATCH_BS(T, hipLaunchKernelGGL(k_search<BS>, dim3(ngrid_search(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
-                                          GE.sid, pos, B.cstart, B.nbr, B.ncnt, T.xcd_map));
+    if ((int64_t)K.nstride * 8 >= (int64_t)1 << 24) {
+        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_search<BS, true>), dim3(ngrid_search(T, K.n)), dim3(BS), 0, st, K,
+                                              GE.geoms, GE.sid, pos, B.cstart, B.nbr, B.ncnt, T.xcd_map));
+    } else {
+        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_search<BS, false>), dim3(ngrid_search(T, K.n)), dim3(BS), 0, st, K,
+                                              GE.geoms, GE.sid, pos, B.cstart, B.nbr, B.ncnt, T.xcd_map));
+    }
     PBF_DISPATCH_BS(T, hipLaunchKernelGGL(k_lambda_first<BS>, dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
                                           GE.sid, pos, B.nbr, B.ncnt, B.lam1, dbg_lam, L, T.xcd_map));
 }

@@ -127,3 +127,21 @@ def test_create_without_gpu_is_an_error_not_a_crash():
     with pytest.raises(PBFError) as ei:
         Solver.from_scene(sc)
     assert ei.value.code == pbf_amd.PBF_ERR_NOGPU
+
+
+def test_create_rejects_too_many_particles():
+    """Particle counts of 2^28 and more are refused before any device work: float4 slots are
+    addressed with 32-bit byte offsets."""
+    from pbf_amd import scenes
+
+    sc = scenes.dam_break((4, 4, 4))
+    n = 1 << 28
+    para = SolverParameter.from_dict(dict(sc.params), n, sc.grid_dims, 0)
+    box = (C.c_float * 6)(*sc.box)
+    bnd = (C.c_float * 6)(*sc.boundary)
+    out = C.c_void_p()
+    lib = pbf_amd.lib()
+    rc = lib.pbf_create(C.byref(para), box, bnd, None, None, n, C.byref(out))
+    assert rc == pbf_amd.PBF_ERR_ARG
+    assert out.value is None
+    assert b"2^28" in lib.pbf_last_error()
