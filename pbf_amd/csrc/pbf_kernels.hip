@@ -431,6 +431,15 @@ __device__ void block_sum2(float c, float d, float* __restrict__ partial) {
 // faces less a margin of 1e-3 h, which covers the rounding of the cell index.
 constexpr int kSub = 8;  // lanes per particle in the search
 
+// b[R] = the value of lane R of this lane's group of 8 (R = 0..RMAX), for two values: a bit-mask
+// swizzle inside 32 lanes, source lane = (lane & 0x18) | R.
+template <int RMAX>
+__device__ __forceinline__ void subgroup_bcast(uint32_t x, uint32_t y, uint32_t* bx, uint32_t* by) {
+    bx[RMAX] = (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (RMAX << 5));
+    by[RMAX] = (uint32_t)__builtin_amdgcn_ds_swizzle((int)y, 0x18 | (RMAX << 5));
+    if constexpr (RMAX > 0) subgroup_bcast<RMAX - 1>(x, y, bx, by);
+}
+
 // WIDE: the table's group stride nstride * 8 needs more than 24 bits (n > 2^21), so list
 // offsets take a full 32-bit multiply instead of the 24-bit one.
 template <int BS, bool WIDE>
@@ -465,12 +474,12 @@ __global__ __launch_bounds__(BS) void k_search(StepConsts K, const SceneGeom* __
                 fh[a] = fmaxf(lo + K.h - p3[a] - margin, 0.0f);
             }
         }
-        // The 9 row runs first (their bounds are independent loads), empty when out of the grid
-        // or culled.
-        uint32_t rb[9], re[9];
-#pragma unroll
-        for (int r = 0; r < 9; ++r) {
-            const int dz = r / 3 - 1, dy = r % 3 - 1;
+        // The 9 row runs (dz, dy) = (r / 3 - 1, r % 3 - 1), empty when out of the grid or culled.
+        // Lane r of the subgroup finds the bounds of run r < 8 and every lane those of run 8; a
+        // swizzle then hands run r's bounds from lane r to the whole subgroup, so the eight lanes
+        // do not each repeat all nine.
+        auto run_bounds = [&](int r, uint32_t& b, uint32_t& e) {
+            const int dz = r / 3 - 1, dy = r - 3 * (r / 3) - 1;
             const int cz = d2 + dz, cy = d1 + dy;
             const float ez = dz < 0 ? fl[2] : (dz > 0 ? fh[2] : 0.0f);
             const float ey = dy < 0 ? fl[1] : (dy > 0 ? fh[1] : 0.0f);
@@ -482,12 +491,19 @@ __global__ __launch_bounds__(BS) void k_search(StepConsts K, const SceneGeom* __
                 if (eyz + fl[0] * fl[0] > K.h2) rlo = d0;
                 if (eyz + fh[0] * fh[0] > K.h2) rhi = d0;
             }
-            // both indices stay inside the scene's cell table even for an unused run
+            // both indices stay inside the scene's cell table even for an unused run; cell
+            // counts stay below 2^28, so their byte offsets fit in 32 bits
             const int row = use ? G.coff + (cz * G.dims[1] + cy) * G.dims[0] : G.coff;
-            const uint32_t lo = cstart[use ? row + rlo : row], hi = cstart[use ? row + rhi + 1 : row];
-            rb[r] = use ? lo : 0u;
-            re[r] = use ? hi : 0u;
-        }
+            const char* cb = reinterpret_cast<const char*>(cstart);
+            const uint32_t lo = *reinterpret_cast<const uint32_t*>(cb + ((uint32_t)(use ? row + rlo : row) << 2));
+            const uint32_t hi = *reinterpret_cast<const uint32_t*>(cb + ((uint32_t)(use ? row + rhi + 1 : row) << 2));
+            b = use ? lo : 0u;
+            e = use ? hi : 0u;
+        };
+        uint32_t mb, me, rb[9], re[9];
+        run_bounds(lane8, mb, me);
+        run_bounds(8, rb[8], re[8]);
+        subgroup_bcast<7>(mb, me, rb, re);
         // Four chunks of 8 candidates per pass: their loads are issued together.  A chunk that
         // holds no candidate for any particle of the wavefront is skipped (a wave-uniform
         // branch), so a run costs ceil(len / 8) chunks, not a multiple of four.
