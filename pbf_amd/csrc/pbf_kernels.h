@@ -13,10 +13,10 @@ namespace pbf {
 // kFarPos) with zero velocity and multiplier, which no kernel writes.  The neighbour list pads
 // each particle's last group of 8 entries with n.
 struct StepBuffers {
-    float4* pos[2];
+    float4* pos[2];      // .w: lambda during the constraint passes, |omega| after the curl pass
     float4* vel[2];
     uint32_t* ids[2];
-    float4* xold;
+    float4* xold;        // step-start positions; after the velocity update, the XSPH sums
     float4* xp;          // xp / vp: staging of host transfers (free during a step)
     float4* vp;
     float4* omega;
@@ -101,7 +101,7 @@ void launch_lambda(hipStream_t st, const StepConsts& K, const Geometry& GE, cons
                    const StepBuffers& B, const LearnArgs& L);
 void launch_delta_apply(hipStream_t st, const StepConsts& K, const Geometry& GE, const Tuning& T, const float4* pos,
                         float4* pos_next, float4* vel, const StepBuffers& B, int last, int first, const LearnArgs& L);
-void launch_curl(hipStream_t st, const StepConsts& K, const Tuning& T, const float4* pos, float4* vel,
+void launch_curl(hipStream_t st, const StepConsts& K, const Tuning& T, float4* pos, const float4* vel,
                  const StepBuffers& B, float* err, float* partial);
 void launch_vort_apply(hipStream_t st, const StepConsts& K, const Tuning& T, const float4* pos, const float4* vel,
                        float4* vel_out, const StepBuffers& B);

@@ -523,8 +523,13 @@ __global__ __launch_bounds__(BS) void k_search(StepConsts K, const SceneGeom* __
                     if (u > 0 && __builtin_amdgcn_ballot_w64(q0 + (uint32_t)(u * kSub) < e) == 0) break;
                     const uint32_t q = q0 + (uint32_t)(u * kSub + lane8);
                     const float rx = xi.x - xs[u].x, ry = xi.y - xs[u].y, rz = xi.z - xs[u].z;
-                    const bool hit = q < e && (int)q != s && rx * rx + ry * ry + rz * rz <= K.h2;
-                    const uint32_t bits = (uint32_t)(__builtin_amdgcn_ballot_w64(hit) >> (kSub * wsub)) & 0xffu;
+                    const float d2 = rx * rx + ry * ry + rz * rz;
+                    // wave masks of the three compares (predicates: 36 unsigned <, 33 !=, 5 ordered
+                    // <=), combined in scalar registers
+                    const uint64_t m = __builtin_amdgcn_uicmp(q, e, 36) & __builtin_amdgcn_uicmp(q, (uint32_t)s, 33) &
+                                       __builtin_amdgcn_fcmpf(d2, K.h2, 5);
+                    const bool hit = q < e && (int)q != s && d2 <= K.h2;
+                    const uint32_t bits = (uint32_t)(m >> (kSub * wsub)) & 0xffu;
                     const int at = cnt + __popc(bits & below);
                     if (hit && at < K.cap) {
                         const uint32_t g = (uint32_t)(at >> 3);
@@ -686,13 +691,20 @@ __global__ __launch_bounds__(BS) void k_delta_apply(StepConsts K, const SceneGeo
     }
 }
 
-// calculateCurl (cu:201-215), plus, with STATS, the density error rho/rho0 - 1 at the final
-// positions into err and / or partial (per workgroup sum |e|, sum max(e,0), max e).
+// calculateCurl (cu:201-215) and the XSPH sum of calculateVorticityAndViscosity (cu:217-239),
+// which need the same neighbour velocities; with STATS also the density error rho/rho0 - 1 at the
+// final positions into err and / or partial (per workgroup sum |e|, sum max(e,0), max e).
+// Outputs for the vorticity pass: omega = (omega, |omega|), |omega| also in pos.w (pos.w is free
+// after the last position update, and the vorticity pass then gathers one float4 per neighbour),
+// and the XSPH sum sum_j (v_j - v_i) W(r) in xsph (the step-start positions, dead once the
+// velocity is updated).  The XSPH term (v_j - v_i) W is the negated (v_i - v_j) W of the
+// reference, and x - (-a) = x + a exactly, so the sum is the reference's bit for bit.  This pass
+// reads only .xyz of its neighbours' positions.
 template <int BS, bool STATS>
-__global__ __launch_bounds__(BS) void k_curl(StepConsts K, const float4* __restrict__ pos, float4* __restrict__ vel,
+__global__ __launch_bounds__(BS) void k_curl(StepConsts K, float4* __restrict__ pos, const float4* __restrict__ vel,
                                              const uint32_t* __restrict__ nbr, const uint8_t* __restrict__ ncnt,
-                                             float4* __restrict__ omega, float* __restrict__ err,
-                                             float* __restrict__ partial, int xcd_map) {
+                                             float4* __restrict__ omega, float4* __restrict__ xsph,
+                                             float* __restrict__ err, float* __restrict__ partial, int xcd_map) {
     const BlockRange R((K.n + BS - 1) / BS, xcd_map);
     float sa = 0.0f, sp = 0.0f, sm = -INFINITY;
     for (int blk = R.first; blk < R.last; ++blk) {
@@ -701,7 +713,7 @@ __global__ __launch_bounds__(BS) void k_curl(StepConsts K, const float4* __restr
         const float4 xi = pos[s];
         const float4 vi = vel[s];
         int cnt = ncnt[s];
-        float wx = 0.0f, wy = 0.0f, wz = 0.0f, rho = 0.0f;
+        float wx = 0.0f, wy = 0.0f, wz = 0.0f, rho = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
         for (int j0 = 0; j0 < cnt; j0 += kGroup) {
             uint32_t qs[kGroup];
             float4 xs[kGroup], vs[kGroup];
@@ -722,14 +734,17 @@ __global__ __launch_bounds__(BS) void k_curl(StepConsts K, const float4* __restr
                 wx = wx + (uy * gz - uz * gy);
                 wy = wy + (uz * gx - ux * gz);
                 wz = wz + (ux * gy - uy * gx);
-                if constexpr (STATS) rho = rho + poly6(r2, K);
+                const float w = poly6(r2, K);
+                px = px + ux * w;
+                py = py + uy * w;
+                pz = pz + uz * w;
+                if constexpr (STATS) rho = rho + w;
             }
         }
         const float wlen = sqrtf(wx * wx + wy * wy + wz * wz);
         omega[s] = make_float4(wx, wy, wz, wlen);
-        // |omega| rides in vel.w for the vorticity pass (its gathers then need pos and vel only);
-        // this pass reads only .xyz of vel.
-        reinterpret_cast<float*>(vel + s)[3] = wlen;
+        reinterpret_cast<float*>(pos + s)[3] = wlen;
+        xsph[s] = make_float4(px, py, pz, 0.0f);
         if constexpr (STATS) {
             float e = rho / K.restRho - 1.0f;
             if (err) err[s] = e;
@@ -761,12 +776,15 @@ __global__ __launch_bounds__(BS) void k_curl(StepConsts K, const float4* __restr
     }
 }
 
-// calculateVorticityAndViscosity (cu:217-239) + applyVorticityAndViscosity (cu:24-29).
+// calculateVorticityAndViscosity (cu:217-239) + applyVorticityAndViscosity (cu:24-29): the
+// vorticity gradient from the neighbours' |omega| (pos.w, left by the curl pass), the XSPH sum
+// from the curl pass.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_vort_apply(StepConsts K, const float4* __restrict__ pos,
                                                    const float4* __restrict__ vel, const float4* __restrict__ omega,
-                                                   const uint32_t* __restrict__ nbr, const uint8_t* __restrict__ ncnt,
-                                                   float4* __restrict__ vel_out, int xcd_map) {
+                                                   const float4* __restrict__ xsph, const uint32_t* __restrict__ nbr,
+                                                   const uint8_t* __restrict__ ncnt, float4* __restrict__ vel_out,
+                                                   int xcd_map) {
     const BlockRange R((K.n + BS - 1) / BS, xcd_map);
     for (int blk = R.first; blk < R.last; ++blk) {
         const int s = blk * BS + threadIdx.x;
@@ -774,34 +792,18 @@ __global__ __launch_bounds__(BS) void k_vort_apply(StepConsts K, const float4* _
         const float4 xi = pos[s];
         const float4 vi = vel[s];
         const float4 wi = omega[s];
+        const float4 p = xsph[s];
         int cnt = ncnt[s];
-        float px = 0.0f, py = 0.0f, pz = 0.0f, ex = 0.0f, ey = 0.0f, ez = 0.0f;
-        for (int j0 = 0; j0 < cnt; j0 += kGroup) {
-            uint32_t qs[kGroup];
-            float4 xs[kGroup], vs[kGroup];
-            load_group(nbr, s, j0, K.nstride, qs);
-#pragma unroll
-            for (int u = 0; u < kGroup; ++u) {
-                xs[u] = slot_ref(pos, qs[u]);
-                vs[u] = slot_ref(vel, qs[u]);
-            }
-#pragma unroll
-            for (int u = 0; u < kGroup; ++u) {
-                const float4 xj = xs[u], vj = vs[u];
-                const float wl = vj.w;  // |omega_j|, left in vel.w by the curl pass
-                float ux = vi.x - vj.x, uy = vi.y - vj.y, uz = vi.z - vj.z;
-                float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-                float r2 = rx * rx + ry * ry + rz * rz;
-                float w = poly6(r2, K);
-                px = px - ux * w;
-                py = py - uy * w;
-                pz = pz - uz * w;
-                float t = spiky_scale(r2, K);
-                ex = ex + wl * (rx * t);
-                ey = ey + wl * (ry * t);
-                ez = ez + wl * (rz * t);
-            }
-        }
+        float ex = 0.0f, ey = 0.0f, ez = 0.0f;
+        for_neighbors(nbr, s, K.nstride, cnt, pos, [&](uint32_t, const float4& xj) {
+            const float wl = xj.w;  // |omega_j|
+            float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+            float r2 = rx * rx + ry * ry + rz * rz;
+            float t = spiky_scale(r2, K);
+            ex = ex + wl * (rx * t);
+            ey = ey + wl * (ry * t);
+            ez = ez + wl * (rz * t);
+        });
         float el = sqrtf(ex * ex + ey * ey + ez * ez);
         if ((double)fabsf(el) > 1e-3) {
             ex = ex / el;
@@ -812,7 +814,7 @@ __global__ __launch_bounds__(BS) void k_vort_apply(StepConsts K, const float4* _
         float cy = ez * wi.x - ex * wi.z;
         float cz = ex * wi.y - ey * wi.x;
         float fx = K.vort_eps * cx, fy = K.vort_eps * cy, fz = K.vort_eps * cz;
-        float vx = K.visc_c * px, vy = K.visc_c * py, vz = K.visc_c * pz;
+        float vx = K.visc_c * p.x, vy = K.visc_c * p.y, vz = K.visc_c * p.z;
         vel_out[s] = make_float4(vi.x + (K.dt * fx + vx), vi.y + (K.dt * fy + vy), vi.z + (K.dt * fz + vz),
                                  0.0f);
     }
@@ -1019,15 +1021,15 @@ void launch_delta_apply(hipStream_t st, const StepConsts& K, const Geometry& GE,
     }
 }
 
-void launch_curl(hipStream_t st, const StepConsts& K, const Tuning& T, const float4* pos, float4* vel,
+void launch_curl(hipStream_t st, const StepConsts& K, const Tuning& T, float4* pos, const float4* vel,
                  const StepBuffers& B, float* err, float* partial) {
     if (K.n <= 0) return;
     if (err || partial) {
         PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_curl<BS, true>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, vel,
-                                              B.nbr, B.ncnt, B.omega, err, partial, T.xcd_map));
+                                              B.nbr, B.ncnt, B.omega, B.xold, err, partial, T.xcd_map));
     } else {
         PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_curl<BS, false>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, vel,
-                                              B.nbr, B.ncnt, B.omega, err, partial, T.xcd_map));
+                                              B.nbr, B.ncnt, B.omega, B.xold, err, partial, T.xcd_map));
     }
 }
 
@@ -1035,7 +1037,7 @@ void launch_vort_apply(hipStream_t st, const StepConsts& K, const Tuning& T, con
                        float4* vel_out, const StepBuffers& B) {
     if (K.n <= 0) return;
     PBF_DISPATCH_BS(T, hipLaunchKernelGGL(k_vort_apply<BS>, dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, vel, B.omega,
-                                          B.nbr, B.ncnt, vel_out, T.xcd_map));
+                                          B.xold, B.nbr, B.ncnt, vel_out, T.xcd_map));
 }
 
 void launch_velocity(hipStream_t st, const StepConsts& K, const float4* pos, float4* vel, const StepBuffers& B) {
