@@ -133,8 +133,8 @@ int pbf_get_params(pbf_solver* s, pbf_params* out);
 /* Block until the solver's queued work is done. */
 int pbf_synchronize(pbf_solver* s);
 
-/* Device view of the cell-sorted state after the last step: float4 (x,y,z,lambda) positions,
- * float4 velocities, uint32 particle ids per slot. */
+/* Device view of the cell-sorted state after the last step: n float4 positions (x, y, z, |curl|
+ * of the step), n float4 velocities (w = 0), uint32 particle ids per slot. */
 typedef struct pbf_device_view {
     const void* pos4;
     const void* vel4;
