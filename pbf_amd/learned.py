@@ -18,9 +18,10 @@
 4. Evaluate on held-out seeds, from identical states (device snapshots): the density error after
    a plain step of I iterations and after a learned step of p passes (a plain pass, the network's
    pass, kept only if mean |C| does not grow, then p - 2 plain iterations).  Error targets are
-   what 8, 16 and 32 plain iterations reach from the same state; for each, the learned step with
-   the fewest passes that reaches it on every evaluated state is timed against the plain step of
-   that many iterations (HIP events over graph-replayed steps).
+   what 8, 16 and 32 plain iterations reach from the same state.  For each target, each method is
+   timed at its fewest iterations / passes that reach the target on every evaluated state (HIP
+   events over graph-replayed steps).  Plain PBF's error need not fall monotonically with I, so
+   the plain time is that of the first I that gets there, which may be below the target's own I.
 """
 from __future__ import annotations
 
@@ -123,7 +124,7 @@ def timed(sc, warmup, steps, iters=None, mlp=None, passes=0):
 
 
 def evaluate(m, seeds, warmup=60, nstates=6, gap=8, n_side=(64, 64, 64), box=(0, 0, 0, 6.4, 4.0, 3.2),
-             passes=(2, 3, 4, 5), plain_iters=(1, 2, 3, 4, 6, 8, 12, 16, 24, 32), time_steps=100, log=print):
+             passes=(2, 3, 4, 5, 6, 8), plain_iters=(1, 2, 3, 4, 6, 8, 12, 16, 24, 32), time_steps=100, log=print):
     rows = []
     for seed in seeds:
         sc = scenes.dam_break(n_side, box, seed=seed)
@@ -163,15 +164,20 @@ def evaluate(m, seeds, warmup=60, nstates=6, gap=8, n_side=(64, 64, 64), box=(0,
     ms_learned = {p: timed(sc, warmup, time_steps, mlp=m, passes=p) for p in passes}
     summary["ms_per_step_learned"] = ms_learned
     summary["targets"] = {}
+    ms_plain = {}
     for T in TARGETS:
         if T not in plain_iters:
             continue
-        best = next((p for p in passes if (lc[p] <= plain_c[T]).all()), None)
-        ms_plain = timed(sc, warmup, time_steps, iters=T)
-        t = {"target": f"compression after {T} plain iterations", "ms_per_step_plain": ms_plain, "learned_passes": best}
+        goal = plain_c[T]  # per state
+        need = min(I for I in plain_iters if (plain_c[I] <= goal).all())  # at most T
+        if need not in ms_plain:
+            ms_plain[need] = timed(sc, warmup, time_steps, iters=need)
+        best = next((p for p in passes if (lc[p] <= goal).all()), None)
+        t = {"target": f"compression after {T} plain iterations", "plain_iters": need,
+             "ms_per_step_plain": ms_plain[need], "learned_passes": best}
         if best is not None:
             t["ms_per_step_learned"] = ms_learned[best]
-            t["time_saving"] = 1.0 - ms_learned[best] / ms_plain
+            t["time_saving"] = 1.0 - ms_learned[best] / ms_plain[need]
         summary["targets"][T] = t
     return rows, summary
 
