@@ -423,9 +423,9 @@ __device__ void block_sum2(float c, float d, float* __restrict__ partial) {
 
 // buildNeighborhood (cu:124-154): the 27 cells in z, y, x order (the three x-neighbours of a row
 // are one contiguous slot run), self excluded, r.r <= h*h, at most cap hits, in that order.
-// Eight lanes work on one particle: they test 8 consecutive candidates of a run at a time (one
-// 128-byte load per subgroup) and a ballot gives each hit its place in the list, so the list
-// keeps the candidates' order.  Cell culling: a neighbour cell whose box lies farther than h
+// Eight lanes work on one particle: they test 8 consecutive candidates of a run at a time (8
+// consecutive positions per load) and the hit masks give each hit its place in the list, so the
+// list keeps the candidates' order.  Cell culling: a neighbour cell whose box lies farther than h
 // from the particle cannot hold a neighbour and is left out of the run (the x-cells that remain
 // are contiguous, so the order is unchanged).  Distances are taken to the particle's own cell
 // faces less a margin of 1e-3 h, which covers the rounding of the cell index.
