@@ -520,15 +520,17 @@ __global__ __launch_bounds__(BS) void k_search(StepConsts K, const SceneGeom* __
                 for (int u = 0; u < kChunks; ++u) xs[u] = slot_ref(pos, min(q0 + (uint32_t)(u * kSub + lane8), e - 1u));
 #pragma unroll
                 for (int u = 0; u < kChunks; ++u) {
-                    if (u > 0 && __builtin_amdgcn_ballot_w64(q0 + (uint32_t)(u * kSub) < e) == 0) break;
                     const uint32_t q = q0 + (uint32_t)(u * kSub + lane8);
+                    // wave masks of the three compares (predicates: 36 unsigned <, 33 !=, 5 ordered
+                    // <=), combined in scalar registers; the run mask also decides the skip
+                    const bool in_run = q < e;
+                    const uint64_t m_run = __builtin_amdgcn_ballot_w64(in_run);
+                    if (u > 0 && m_run == 0) break;
                     const float rx = xi.x - xs[u].x, ry = xi.y - xs[u].y, rz = xi.z - xs[u].z;
                     const float d2 = rx * rx + ry * ry + rz * rz;
-                    // wave masks of the three compares (predicates: 36 unsigned <, 33 !=, 5 ordered
-                    // <=), combined in scalar registers
-                    const uint64_t m = __builtin_amdgcn_uicmp(q, e, 36) & __builtin_amdgcn_uicmp(q, (uint32_t)s, 33) &
-                                       __builtin_amdgcn_fcmpf(d2, K.h2, 5);
-                    const bool hit = q < e && (int)q != s && d2 <= K.h2;
+                    const bool other = q != (uint32_t)s, close = d2 <= K.h2;
+                    const uint64_t m = m_run & __builtin_amdgcn_ballot_w64(other) & __builtin_amdgcn_fcmpf(d2, K.h2, 5);
+                    const bool hit = in_run && other && close;
                     const uint32_t bits = (uint32_t)(m >> (kSub * wsub)) & 0xffu;
                     const int at = cnt + __popc(bits & below);
                     if (hit && at < K.cap) {
