@@ -80,6 +80,18 @@ def test_one_step_1m():
     compare_step(g, o, check_lists=False)
 
 
+def test_one_step_wide_list_offsets():
+    """2,129,920 particles: the neighbour table's group stride (nstride * 8) needs more than 24
+    bits, so the search stores list entries through its full 32-bit multiply.  Grid, lists and
+    state bit for bit."""
+    sc = scenes.dam_break((130, 128, 128), (0, 0, 0, 13.0, 6.4, 6.4), seed=9)
+    assert (sc.n + 63) // 64 * 64 * 8 >= 1 << 24
+    g, o = pair(sc)
+    g.step()
+    o.step(1)
+    compare_step(g, o)
+
+
 def test_many_steps_32k_bitexact():
     g, o = pair(scenes.config_cpu_32k(), flags=FLAG_DENSITY_STATS)
     for _ in range(4):
@@ -122,6 +134,41 @@ def test_iteration_counts(iters):
     o.step(1)
     # with no iteration the reference computes no lambda (cudaPBF.cu:329)
     compare_step(g, o, check_lambda=iters > 0)
+
+
+def test_scorr_integer_exponent_generic_path():
+    """An s_corr exponent other than 4 (here 3) runs the generic delta-p kernel, which raises the
+    ratio by repeated multiplication: still bit for bit over several steps."""
+    g, o = pair(scenes.config_cpu_32k(), n=3.0)
+    for _ in range(3):
+        g.step()
+        o.step(1)
+    compare_step(g, o)
+
+
+def test_scorr_fractional_exponent():
+    """A non-integer exponent (2.5) takes powf on the device against std::pow on the host, which
+    may differ in the last place (parity unpinned at that call).  Grid, order, lists and the
+    first lambda (which has no s_corr) are exact.  The first step from a lattice amplifies
+    last-place differences in s_corr: changing every pow result of the oracle itself by one ulp
+    moves positions by up to 1e-3 m, with 90% of coordinates within 6e-8 m and 99% within 7e-6 m.
+    The bounds below sit on that measured sensitivity."""
+    g, o = pair(scenes.config_cpu_32k(), n=2.5)
+    g.step()
+    o.step(1)
+    gg, og = g.grid(), o.grid()
+    assert np.array_equal(gg[0], og[0]) and np.array_equal(gg[3][: og[4]], og[3][: og[4]])
+    gn, gc = g.neighbors()
+    on, oc = o.neighbors()
+    assert np.array_equal(gc, oc) and np.array_equal(gn, on), "neighbour lists"
+    assert_bitexact(g.fields()[0], o.fields()[0], "lambda first")
+    gp, gv = g.get_state()
+    op, ov = o.get_state()
+    d = np.abs(gp.astype(np.float64) - op)
+    assert np.percentile(d, 90) <= 1e-6 and np.percentile(d, 99) <= 1e-4 and d.max() <= 5e-3, (
+        np.percentile(d, 90), np.percentile(d, 99), d.max())
+    np.testing.assert_allclose(gp.mean(0), op.mean(0), rtol=0, atol=1e-6)
+    np.testing.assert_allclose(gv.mean(0), ov.mean(0), rtol=0, atol=1e-4)
 
 
 def test_reference_grid_64_cubed():
