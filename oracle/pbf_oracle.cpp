@@ -212,6 +212,10 @@ inline int dim_of(float p, float pmin, float invH, int dim) {
 extern "C" {
 
 float orc_sin(float x) { return pbf_sin(x); }
+
+void orc_rsqrt(const float* x, float* y, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) y[i] = pbf_rsqrt(x[i]);
+}
 float orc_random_offset(float u, float v) { return random_offset(u, v); }
 
 void orc_bound(float* p3, const float boundary[6], float coeff) {

@@ -42,6 +42,7 @@ def orc() -> C.CDLL:
         L = C.CDLL(ORACLE)
         sig = {
             "orc_sin": (C.c_float, [C.c_float]),
+            "orc_rsqrt": (None, [_F, _F, C.c_int64]),
             "orc_random_offset": (C.c_float, [C.c_float, C.c_float]),
             "orc_bound": (None, [_F, _F, C.c_float]),
             "orc_consts": (None, [C.POINTER(SolverParameter), _F]),
