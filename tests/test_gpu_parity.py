@@ -171,6 +171,18 @@ def test_scorr_fractional_exponent():
     np.testing.assert_allclose(gv.mean(0), ov.mean(0), rtol=0, atol=1e-4)
 
 
+def test_large_grid_three_launch_scan():
+    """A grid of more than 2048 scan tiles (160 x 160 x 96 = 2,457,600 cells) takes the
+    three-launch scan (tile totals, their scan, the down-sweep): grid, lists and state bit for
+    bit."""
+    sc = scenes.dam_break((32, 32, 32), (0, 0, 0, 16.0, 16.0, 9.6), seed=10)
+    assert sc.grid_dims == (160, 160, 96)
+    g, o = pair(sc)
+    g.step()
+    o.step(1)
+    compare_step(g, o)
+
+
 def test_reference_grid_64_cubed():
     """The reference's literal 64^3 grid (solverBase.h:11-13) anchored at the box minimum."""
     sc = scenes.dam_break((32, 32, 32), (0, 0, 0, 6.4, 6.4, 6.4), seed=3)
