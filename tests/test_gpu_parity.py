@@ -224,6 +224,28 @@ def test_neighbour_cap_cluster():
     compare_step(g, o)
 
 
+@pytest.mark.parametrize("cap", [12, 100, 255])
+def test_neighbour_cap_values(cap):
+    """Caps other than the reference's 63: a list table of ceil(cap / 8) groups, truncation at
+    the cap inside a dense cluster (about 200 particles within 0.04 m), and padding of the last
+    group.  The lists of the first step (cluster intact), then the state after a second step,
+    bit for bit."""
+    rng = np.random.default_rng(1)
+    sc = scenes.dam_break((10, 10, 10), (0, 0, 0, 1.6, 1.6, 1.6), seed=11)
+    pos = sc.pos.copy()
+    pos[:200] = (np.array([0.8, 0.8, 0.8]) + rng.uniform(-0.02, 0.02, size=(200, 3))).astype(np.float32)
+    sc.pos = pos
+    g, o = pair(sc, max_neighbors=cap)
+    g.step()
+    o.step(1)
+    on, oc = o.neighbors()
+    assert oc.max() == min(cap, 199)
+    compare_step(g, o)
+    g.step()
+    o.step(1)
+    compare_step(g, o)
+
+
 def test_tiny_and_empty():
     sc = scenes.dam_break((1, 1, 1), (0, 0, 0, 1.6, 1.6, 1.6), seed=7)
     g, o = pair(sc)
