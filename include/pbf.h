@@ -245,10 +245,11 @@ int pbf_last_guess_info(pbf_solver* s, float out[5]);
  * steps with params.iterations iterations again. */
 int pbf_set_learned(pbf_solver* s, int32_t passes);
 
-/* Measurement: the average duration of one launch of a constraint kernel (stage 0: neighbour
- * build + iteration-1 lambda, 1: lambda, 2: delta-p + apply), replayed reps times back to back
- * on the state of the last step, between HIP events on the solver's stream.  The state is
- * unchanged (the replays write only what the next step rebuilds). */
+/* Measurement: the average duration of one launch of a step stage (stage 0: neighbour build +
+ * iteration-1 lambda, 1: lambda, 2: delta-p + apply, 3: predict + hash, scan, scatter + rank +
+ * reorder), replayed reps times back to back on the state of the last step, between HIP events
+ * on the solver's stream.  The state is unchanged (the replays write only what the next step
+ * rebuilds). */
 int pbf_time_stage(pbf_solver* s, int32_t stage, int32_t reps, float* ms);
 
 /* save != 0: copy the current state (positions, velocities, particle order) into one device-side

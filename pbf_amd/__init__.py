@@ -362,11 +362,12 @@ class Solver:
         _check(lib().pbf_time_steps(self._h, int(nsteps), _fp(ms)))
         return float(ms[0])
 
-    STAGES = {"neighbors_lambda": 0, "lambda": 1, "delta_apply": 2}
+    STAGES = {"neighbors_lambda": 0, "lambda": 1, "delta_apply": 2, "sort_reorder": 3}
 
     def time_stage(self, stage: str, reps: int = 50) -> float:
-        """ms per launch of one constraint kernel ("neighbors_lambda", "lambda", "delta_apply"),
-        replayed back to back on the last step's state (state unchanged)."""
+        """ms per launch of one step stage ("neighbors_lambda", "lambda", "delta_apply", or
+        "sort_reorder" = predict + hash, scan, scatter + rank + reorder), replayed back to back
+        on the last step's state (state unchanged)."""
         ms = np.zeros(1, np.float32)
         _check(lib().pbf_time_stage(self._h, self.STAGES[stage], int(reps), _fp(ms)))
         return float(ms[0])

@@ -804,33 +804,54 @@ int pbf_profile_step(pbf_solver* s, float* ms, const char** names, int32_t max) 
 }
 
 
-// Average duration of one launch of a constraint kernel (0: neighbour build + lambda 1,
-// 1: lambda, 2: delta-p + apply), replayed reps times back to back on the state of the last step
-// between two HIP events on the solver's stream.  The replays write only buffers that the next
-// step rebuilds (lists, lambda in pos.w, the free position buffer), so the state is unchanged.
+// Average duration of one launch of a step stage (0: neighbour build + lambda 1, 1: lambda,
+// 2: delta-p + apply, 3: predict + hash, scan, scatter + rank + reorder), replayed reps times
+// back to back on the state of the last step between two HIP events on the solver's stream.
+// The replays write only buffers that the next step rebuilds (cell tables, lists, lambda in
+// pos.w, the free position / velocity / id buffers, x_old; the scan clears the cell counters
+// again), so the state is unchanged.
 int pbf_time_stage(pbf_solver* s, int32_t stage, int32_t reps, float* ms) {
-    if (!s || !ms || reps < 1 || stage < 0 || stage > 2) return set_error(PBF_ERR_ARG, "bad argument");
+    if (!s || !ms || reps < 1 || stage < 0 || stage > 3) return set_error(PBF_ERR_ARG, "bad argument");
     if (s->steps_done < 1 || s->n == 0) return set_error(PBF_ERR_STATE, "run a step first");
     HIP_TRY(hipSetDevice(s->device));
     const StepConsts K = make_consts(s);
     StepBuffers& B = s->B;
     const int a = s->pc;
     const LearnArgs L0{};
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    HIP_TRY(hipEventRecord(e0, s->stream));
+    // the reps replays are captured in one graph, as a step runs, so host launch overhead does
+    // not enter short stages; one untimed launch of the graph warms up
+    HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     for (int r = 0; r < reps; ++r) {
         if (stage == 0) launch_neighbors_lambda(s->stream, K, s->geo, s->tune, B.pos[a], B, nullptr, L0);
         if (stage == 1) launch_lambda(s->stream, K, s->geo, s->tune, B.pos[a], B, L0);
         if (stage == 2) launch_delta_apply(s->stream, K, s->geo, s->tune, B.pos[a], B.pos[a ^ 1], B.vel[s->vc], B, 0, 1, L0);
+        if (stage == 3) {
+            launch_predict_hash(s->stream, K, s->geo, B.pos[s->pc], B.vel[s->vc], B);
+            launch_scan(s->stream, K, B);
+            launch_sort_reorder(s->stream, K, s->geo, B, B.pos[s->pc], B.vel[s->vc], B.ids[s->ic], B.pos[s->pc ^ 1],
+                                B.vel[s->vc ^ 1], B.ids[s->ic ^ 1], nullptr);
+        }
     }
-    HIP_TRY(hipEventRecord(e1, s->stream));
-    HIP_TRY(hipEventSynchronize(e1));
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamEndCapture(s->stream, &g));
+    hipGraphExec_t ge = nullptr;
+    hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return set_error(PBF_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     float t = 0.0f;
-    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    e = hipGraphLaunch(ge, s->stream);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, s->stream);
+    if (e == hipSuccess) e = hipGraphLaunch(ge, s->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, s->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipGraphExecDestroy(ge);
+    if (e != hipSuccess) return set_error(PBF_ERR_HIP, "stage timing: %s", hipGetErrorString(e));
     HIP_TRY(hipGetLastError());
     s->lam_last = -1;
     *ms = t / (float)reps;

@@ -141,6 +141,26 @@ def test_parameter_changes_between_graph_replays():
     assert_bitexact(va, vb, "velocities")
 
 
+def test_stage_replays_leave_the_state_unchanged():
+    """pbf_time_stage replays a stage on the last step's state (the sort stage included, which
+    rewrites the spare position / velocity / id buffers and the cell tables): afterwards the
+    solver agrees bit for bit with an untouched twin, now and two steps later."""
+    sc = scenes.dam_break((20, 20, 20), (0, 0, 0, 1.6, 1.6, 1.6), seed=13)
+    a = Solver.from_scene(sc)
+    b = Solver.from_scene(sc)
+    a.advance(3)
+    b.advance(3)
+    for stage in Solver.STAGES:
+        assert a.time_stage(stage, 5) > 0.0
+    for k in range(2):
+        pa, va = a.get_state()
+        pb, vb = b.get_state()
+        assert_bitexact(pa, pb, f"positions ({k})")
+        assert_bitexact(va, vb, f"velocities ({k})")
+        a.advance(2)
+        b.advance(2)
+
+
 def test_literal_parameters_one_step():
     """The physical constants passed unchanged (a diverging configuration) still match."""
     sc = scenes.config_cpu_32k()
