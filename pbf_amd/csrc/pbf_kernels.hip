@@ -37,7 +37,8 @@ constexpr int kXcds = 8;  // MI355X: workgroups are dispatched round-robin over 
 
 // Workgroup -> chunk index: the workgroups one XCD receives (blockIdx.x = x, x+8, x+16, ...)
 // take one contiguous range of chunks, so that range's neighbour gathers stay in that XCD's
-// L2.  A bijection on [0, gridDim.x).
+// L2, and the sort kernels' cell counters, buckets and gathers of nearby slots are touched
+// from one XCD rather than eight.  A bijection on [0, gridDim.x).
 __device__ __forceinline__ int xcd_block() {
     const int b = blockIdx.x, nb = gridDim.x;
     const int q = nb / kXcds, r = nb % kXcds, x = b % kXcds;
@@ -107,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_predict_hash(StepConsts K, const Sce
                                                          const uint8_t* __restrict__ sid, const float4* __restrict__ pos,
                                                          const float4* __restrict__ vel, uint32_t* __restrict__ cell,
                                                          uint32_t* __restrict__ local, uint32_t* __restrict__ count) {
-    int s = blockIdx.x * kBlock + threadIdx.x;
+    int s = xcd_block() * kBlock + threadIdx.x;
     if (s >= K.n) return;  // whole tail lanes leave; ballots below count the active lanes only
     const SceneGeom G = geom_of(K, geoms, sid, s);
     float4 x = pos[s];
@@ -284,7 +285,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(int n, const uint32_t* __res
                                                     const uint32_t* __restrict__ cstart,
                                                     const uint32_t* __restrict__ ids_in, uint32_t* __restrict__ bucket,
                                                     uint32_t* __restrict__ bucket_id) {
-    int s = blockIdx.x * kBlock + threadIdx.x;
+    int s = xcd_block() * kBlock + threadIdx.x;
     if (s >= n) return;
     const uint32_t d = cstart[cell[s]] + local[s];
     bucket[d] = (uint32_t)s;
@@ -304,7 +305,7 @@ __global__ __launch_bounds__(kBlock) void k_rank_reorder(StepConsts K, const Sce
                                                          const float4* __restrict__ pos_in, float4* __restrict__ pos_out,
                                                          float4* __restrict__ vel_out, float4* __restrict__ xold_out,
                                                          uint32_t* __restrict__ ids_out, int32_t* __restrict__ dbg_gid) {
-    int t = blockIdx.x * kBlock + threadIdx.x;
+    int t = xcd_block() * kBlock + threadIdx.x;
     if (t >= K.n) return;
     const uint32_t e = bucket[t];
     const uint32_t id = bucket_id[t];
