@@ -31,7 +31,10 @@ constexpr int kB = 256;
 constexpr int LD64 = 72;  // LDS row pitch (bf16) of 64-wide rows: 144 B spreads the banks
 constexpr int LD16 = 24;  // ... of 16-wide rows
 
-__device__ __forceinline__ float sigmoidf(float z) { return 1.0f / (1.0f + __expf(-z)); }
+// The network computes in BF16 (activations are rounded to 8 significant bits right after), so
+// the sigmoid takes the hardware exp and reciprocal (about 1 ulp) rather than the
+// correctly-rounded division this library is otherwise built with.
+__device__ __forceinline__ float sigmoidf(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 __device__ __forceinline__ float siluf(float z) { return z * sigmoidf(z); }
 __device__ __forceinline__ float dsiluf(float z) {
     float s = sigmoidf(z);
