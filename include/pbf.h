@@ -249,7 +249,8 @@ int pbf_set_learned(pbf_solver* s, int32_t passes);
  * iteration-1 lambda, 1: lambda, 2: delta-p + apply, 3: predict + hash, scan, scatter + rank +
  * reorder), replayed reps times back to back on the state of the last step, between HIP events
  * on the solver's stream.  The state is unchanged (the replays write only what the next step
- * rebuilds). */
+ * rebuilds); the diagnostics of the last step (pbf_get_grid, pbf_get_neighbors, pbf_get_fields)
+ * no longer describe it afterwards. */
 int pbf_time_stage(pbf_solver* s, int32_t stage, int32_t reps, float* ms);
 
 /* save != 0: copy the current state (positions, velocities, particle order) into one device-side
