@@ -124,7 +124,7 @@ def timed(sc, warmup, steps, iters=None, mlp=None, passes=0):
 
 
 def evaluate(m, seeds, warmup=60, nstates=6, gap=8, n_side=(64, 64, 64), box=(0, 0, 0, 6.4, 4.0, 3.2),
-             passes=(2, 3, 4, 5, 6, 8), plain_iters=(1, 2, 3, 4, 6, 8, 12, 16, 24, 32), time_steps=100, log=print):
+             passes=(2, 3, 4, 5, 6, 7, 8), plain_iters=(1, 2, 3, 4, 6, 8, 12, 16, 24, 32), time_steps=100, log=print):
     rows = []
     for seed in seeds:
         sc = scenes.dam_break(n_side, box, seed=seed)
