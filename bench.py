@@ -112,7 +112,7 @@ def learned_leg(n_side, box, train_seeds, eval_seeds, rec_steps, stride, warm):
     mean, inv, ym, ys, nvalid = learned.normalise(ds, rows)
     m, losses = learned.train(ds, nvalid, mean, inv, ym, ys, steps=3000, log=lambda *a: None)
     ds.close()
-    _, summ = learned.evaluate(m, list(eval_seeds), warm, nstates=4, n_side=n_side, box=box, time_steps=100, log=lambda *a: None)
+    _, summ = learned.evaluate(m, list(eval_seeds), warm, nstates=4, n_side=n_side, box=box, time_steps=300, log=lambda *a: None)
     summ["samples"] = nvalid
     summ["loss"] = [float(losses[:20].mean()), float(losses[-100:].mean())]
     m.close()
