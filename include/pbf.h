@@ -157,7 +157,9 @@ int pbf_get_grid(pbf_solver* s, int32_t* gid, int32_t* count, int32_t* cell_star
 int pbf_get_neighbors(pbf_solver* s, int32_t* nbr, int32_t* cnt);
 
 /* lambda of the first and last Jacobi iteration, density error rho/restRho-1 at the end of the
- * constraint loop, curl (xyz) -- all by particle id; any pointer may be NULL. */
+ * constraint loop, curl (xyz) -- all by particle id; any pointer may be NULL.  After a learned
+ * step of 2 passes whose guess was rejected, the last lambda is the network's multiplier (the
+ * positions are those of the plain pass). */
 int pbf_get_fields(pbf_solver* s, float* lam_first, float* lam_last, float* err, float* curl_xyz);
 
 /* Density-error statistics of the last step: out[0] = mean |err|, out[1] = mean max(err,0),

@@ -204,9 +204,9 @@ int prof_mark(pbf_solver* s, const char* name) {
 // from the feature lambda pass, lambda 2 in .w (a copy in lam_keep) and the residual partials
 // resid0.  The network's multiplier (from the features) replaces lambda 2 and one delta-p pass
 // runs on it.  The gate keeps the result only if mean |C| did not grow over the state after pass
-// 1; otherwise lambda 2 is restored and the pass reruns plain (both fallback launches return at
-// once when the guess was kept).  Leaves the positions after the pass in pos[a ^ 1] and, with
-// need_lambda, their lambda in .w and residual partials in resid.
+// 1; otherwise the pass reruns plain, its delta-p reading lambda 2 from lam_keep (both fallback
+// launches return at once when the guess was kept).  Leaves the positions after the pass in
+// pos[a ^ 1] and, with need_lambda, their lambda in .w and residual partials in resid.
 int enqueue_guess_pass(pbf_solver* s, const StepConsts& K, int a, int v, bool need_lambda) {
     StepBuffers& B = s->B;
     hipStream_t st = s->stream;
@@ -220,13 +220,15 @@ int enqueue_guess_pass(pbf_solver* s, const StepConsts& K, int a, int v, bool ne
     launch_lambda(st, K, s->geo, s->tune, B.pos[a ^ 1], B, LR);
     PROF("lambda");
     launch_guess_gate(st, s->resid0, s->resid, reduce_blocks(s->tune, s->n), s->n, s->gate, s->gate_stats);
-    launch_restore_lambda(st, s->gate, s->lam_keep, B.pos[a], s->n);
     LearnArgs LS{};
     LS.skip = s->gate;
+    LS.lam_src = s->lam_keep;
     launch_delta_apply(st, K, s->geo, s->tune, B.pos[a], B.pos[a ^ 1], B.vel[v], B, 0, 0, LS);
     if (need_lambda) {
-        LS.resid = s->resid;
-        launch_lambda(st, K, s->geo, s->tune, B.pos[a ^ 1], B, LS);
+        LearnArgs LL{};
+        LL.skip = s->gate;
+        LL.resid = s->resid;
+        launch_lambda(st, K, s->geo, s->tune, B.pos[a ^ 1], B, LL);
     }
     PROF("guess_gate_fallback");
     HIP_TRY(hipGetLastError());
@@ -423,7 +425,7 @@ int create_solver(const pbf_params* para, const float* grid_space, const float* 
         (rc = dalloc(s, &B.lam1, np)))
         return fail(rc);
     s->npartial = (int)((np + 63) / 64) + 1;  // enough for the smallest workgroup size
-    if ((rc = dalloc(s, &s->feat, np * kNumFeatures)) || (rc = dalloc(s, &s->lam_sum, np)) || (rc = dalloc(s, &s->lam_keep, np)) ||
+    if ((rc = dalloc(s, &s->feat, np * kNumFeatures)) || (rc = dalloc(s, &s->lam_sum, np)) || (rc = dalloc(s, &s->lam_keep, np + 1)) ||
         (rc = dalloc(s, &s->resid, (size_t)s->npartial * 2)) ||
         (rc = dalloc(s, &s->resid0, (size_t)s->npartial * 2)) || (rc = dalloc(s, &s->gate, 4)) ||
         (rc = dalloc(s, &s->gate_stats, 4)))
@@ -446,6 +448,7 @@ int create_solver(const pbf_params* para, const float* grid_space, const float* 
     }
     launch_fill_u32(s->stream, B.count, mpad, 0u);
     launch_fill_u32(s->stream, reinterpret_cast<uint32_t*>(s->gate), 4, 0u);
+    launch_fill_u32(s->stream, reinterpret_cast<uint32_t*>(s->lam_keep), np + 1, 0u);  // entry n: the padding slot
     launch_fill_u32(s->stream, reinterpret_cast<uint32_t*>(s->gate_stats), 4, 0u);
     if (hipGetLastError() != hipSuccess) return fail(set_error(PBF_ERR_HIP, "fill launch failed"));
     *out = s;

@@ -62,6 +62,7 @@ struct LearnArgs {
     float* lam_sum;      // accumulated multiplier of this step (delta-p passes)
     float* resid;        // per-workgroup (sum max(C, 0), sum |C|) pairs (lambda passes)
     const int* skip;     // delta-p / lambda passes return at once when *skip != 0
+    const float* lam_src;  // delta-p: multipliers from here (n + 1 entries) instead of pos.w
 };
 
 // Per-slot scene geometry of a batch (both NULL for a single scene).
@@ -105,9 +106,8 @@ void launch_curl(hipStream_t st, const StepConsts& K, const Tuning& T, float4* p
                  const StepBuffers& B, float* err, float* partial);
 void launch_vort_apply(hipStream_t st, const StepConsts& K, const Tuning& T, const float4* pos, const float4* vel,
                        float4* vel_out, const StepBuffers& B);
-// learned first pass: accept / reject on the device, and the fallback's lambda restore
+// learned pass: accept / reject on the device
 void launch_guess_gate(hipStream_t st, const float* r0, const float* rg, int np, int n, int* gate, float* stats);
-void launch_restore_lambda(hipStream_t st, const int* gate, const float* lam, float4* pos, int n);
 void launch_velocity(hipStream_t st, const StepConsts& K, const float4* pos, float4* vel, const StepBuffers& B);
 void launch_export(hipStream_t st, int n, const float4* pos, const float4* vel, const uint32_t* ids, float* pos_xyz,
                    float* vel_xyz);

@@ -655,7 +655,10 @@ __global__ __launch_bounds__(BS) void k_lambda(StepConsts K, const SceneGeom* __
 // calculateOffset (cu:182-199) + updatePosition (cu:15-22): delta-p with s_corr, then
 // x + delta-p clamped into the next position buffer; the last iteration also writes
 // v = (x - x_old) / dt.  With LA.lam_sum the multiplier of this iteration is accumulated.
-template <int BS, int SN>
+// LS: the multipliers come from LA.lam_src (n + 1 entries, the padding slot's 0 included)
+// instead of pos.w: the fallback pass after a rejected learned guess reads the kept lambda 2
+// there, so pos.w (the guess) needs no restore.
+template <int BS, int SN, bool LS>
 __global__ __launch_bounds__(BS) void k_delta_apply(StepConsts K, const SceneGeom* __restrict__ geoms,
                                                     const uint8_t* __restrict__ sid, const float4* __restrict__ pos,
                                                     float4* __restrict__ pos_next, const uint32_t* __restrict__ nbr,
@@ -668,15 +671,17 @@ __global__ __launch_bounds__(BS) void k_delta_apply(StepConsts K, const SceneGeo
         const int s = blk * BS + threadIdx.x;
         if (s >= K.n) continue;
         const float4 xi = pos[s];
+        const float lam_i = LS ? LA.lam_src[s] : xi.w;
         int cnt = ncnt[s];
-        if (LA.lam_sum) LA.lam_sum[s] = (first ? 0.0f : LA.lam_sum[s]) + xi.w;
+        if (LA.lam_sum) LA.lam_sum[s] = (first ? 0.0f : LA.lam_sum[s]) + lam_i;
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
-        for_neighbors(nbr, s, K.nstride, cnt, pos, [&](uint32_t, const float4& xj) {
+        for_neighbors(nbr, s, K.nstride, cnt, pos, [&](uint32_t q, const float4& xj) {
             float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
             float r2 = rx * rx + ry * ry + rz * rz;
             float w = poly6(r2, K);
             float t = spiky_scale(r2, K);
-            float a = (xi.w + xj.w) + (SN == 4 ? scorr4(w, K) : scorr(w, K));
+            const float lam_j = LS ? LA.lam_src[q] : xj.w;
+            float a = (lam_i + lam_j) + (SN == 4 ? scorr4(w, K) : scorr(w, K));
             ax = ax + a * (rx * t);
             ay = ay + a * (ry * t);
             az = az + a * (rz * t);
@@ -855,21 +860,11 @@ __global__ __launch_bounds__(256) void k_guess_gate(const float* __restrict__ r0
     }
 }
 
-// Rejected guess: put the saved lambda back into pos.w (no-op when the guess was kept).
-__global__ __launch_bounds__(kBlock) void k_restore_lambda(const int* __restrict__ gate, const float* __restrict__ lam,
-                                                           float4* __restrict__ pos, int n) {
-    if (*gate) return;
-    const int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s < n) reinterpret_cast<float*>(pos + s)[3] = lam[s];
-}
 
 void launch_guess_gate(hipStream_t st, const float* r0, const float* rg, int np, int n, int* gate, float* stats) {
     hipLaunchKernelGGL(k_guess_gate, dim3(1), dim3(256), 0, st, r0, rg, np, n > 0 ? 1.0f / (float)n : 0.0f, gate, stats);
 }
 
-void launch_restore_lambda(hipStream_t st, const int* gate, const float* lam, float4* pos, int n) {
-    if (n > 0) hipLaunchKernelGGL(k_restore_lambda, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, gate, lam, pos, n);
-}
 
 // updatePosition's velocity (cu:21) for a constraint loop that stopped early.
 __global__ __launch_bounds__(kBlock) void k_velocity(StepConsts K, const float4* __restrict__ pos,
@@ -1013,12 +1008,16 @@ void launch_lambda(hipStream_t st, const StepConsts& K, const Geometry& GE, cons
 void launch_delta_apply(hipStream_t st, const StepConsts& K, const Geometry& GE, const Tuning& T, const float4* pos,
                         float4* pos_next, float4* vel, const StepBuffers& B, int last, int first, const LearnArgs& L) {
     if (K.n <= 0) return;
-    if (K.n_int == 4) {
-        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_delta_apply<BS, 4>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K,
+    if (L.lam_src) {  // the fallback after a rejected learned guess (rare): the generic s_corr kernel
+        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_delta_apply<BS, 0, true>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K,
+                                              GE.geoms, GE.sid, pos, pos_next, B.nbr, B.ncnt, B.xold, vel, last, first,
+                                              L, T.xcd_map));
+    } else if (K.n_int == 4) {
+        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_delta_apply<BS, 4, false>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K,
                                               GE.geoms, GE.sid, pos, pos_next, B.nbr, B.ncnt, B.xold, vel, last, first,
                                               L, T.xcd_map));
     } else {
-        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_delta_apply<BS, 0>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K,
+        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_delta_apply<BS, 0, false>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K,
                                               GE.geoms, GE.sid, pos, pos_next, B.nbr, B.ncnt, B.xold, vel, last, first,
                                               L, T.xcd_map));
     }
