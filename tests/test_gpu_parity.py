@@ -291,6 +291,21 @@ def test_neighbour_cap_values(cap):
     compare_step(g, o)
 
 
+def test_shuffled_input_order():
+    """Particles given in random order: in the first step each workgroup of the scatter holds
+    particles of many scan tiles (16^3 cells + 1 = 5 tiles of 1024 counters), which it scans one
+    after another; the second step starts from cell order again.  Both steps bit for bit."""
+    sc = scenes.config_cpu_32k(seed=9)
+    perm = np.random.default_rng(3).permutation(sc.n)
+    sc.pos = np.ascontiguousarray(sc.pos[perm])
+    sc.vel = np.ascontiguousarray(sc.vel[perm])
+    g, o = pair(sc)
+    for _ in range(2):
+        g.step()
+        o.step(1)
+        compare_step(g, o)
+
+
 def test_tiny_and_empty():
     sc = scenes.dam_break((1, 1, 1), (0, 0, 0, 1.6, 1.6, 1.6), seed=7)
     g, o = pair(sc)
@@ -300,10 +315,12 @@ def test_tiny_and_empty():
     e = scenes.dam_break((1, 1, 1), (0, 0, 0, 1.6, 1.6, 1.6), seed=7)
     e.pos = e.pos[:0]
     e.vel = e.vel[:0]
-    s = Solver.from_scene(e)
+    s = Solver.from_scene(e, flags=FLAG_DEBUG)
     s.step()
     p, v = s.get_state()
     assert p.shape == (0, 3)
+    _, cnt, cs, _, nin = s.grid()
+    assert nin == 0 and not cnt.any() and not cs.any(), "empty cell table"
 
 
 def test_density_stats_match_oracle():
