@@ -233,6 +233,9 @@ def main():
     sort_ms_events = sum(per_step_ms.get(k, 0.0) for k in sort_stage)
     sort_ms = s.time_stage("sort_reorder", 50)  # back-to-back replays, as the step runs them
     sort_bytes = float(n * 96 + 12 * ncells)
+    # hash, scan (2 launches up to 2048 tiles of 1024 cell counters, else 3), scatter, reorder
+    sort_launches = 3 + (2 if ncells + 1 <= 2048 * 1024 else 3)
+    floor_ms = s.time_stage("launch_floor", 50)  # an empty kernel of the same grid, replayed
 
     pos, vel = s.get_state()
     finite = bool(np.isfinite(pos).all() and np.isfinite(vel).all())
@@ -291,7 +294,9 @@ def main():
         "stages_ms": {k: round(v, 5) for k, v in per_step_ms.items()},
         "step_roofline_frac": step_bytes(n, ncells, iters) / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9),
         "sort_reorder": {"ms": sort_ms, "frac": sort_bytes / (sort_ms * 1e-3) / (HBM_PEAK_GBS * 1e9) if sort_ms else None,
-                         "ms_events_eager": sort_ms_events},
+                         "ms_events_eager": sort_ms_events, "launches": sort_launches,
+                         "launch_floor_ms": floor_ms,
+                         "launch_floor_share": sort_launches * floor_ms / sort_ms if sort_ms else None},
         "copy_kernel_gbs": copy_gbs,
         "state_finite": finite,
         "cpu_baseline": cpu,

@@ -362,12 +362,13 @@ class Solver:
         _check(lib().pbf_time_steps(self._h, int(nsteps), _fp(ms)))
         return float(ms[0])
 
-    STAGES = {"neighbors_lambda": 0, "lambda": 1, "delta_apply": 2, "sort_reorder": 3}
+    STAGES = {"neighbors_lambda": 0, "lambda": 1, "delta_apply": 2, "sort_reorder": 3, "launch_floor": 4}
 
     def time_stage(self, stage: str, reps: int = 50) -> float:
-        """ms per launch of one step stage ("neighbors_lambda", "lambda", "delta_apply", or
-        "sort_reorder" = predict + hash, scan, scatter + rank + reorder), replayed back to back
-        on the last step's state (state unchanged)."""
+        """ms per launch of one step stage ("neighbors_lambda", "lambda", "delta_apply",
+        "sort_reorder" = predict + hash, scan, scatter + rank + reorder, or "launch_floor" = an
+        empty kernel of the particle grid's shape), replayed back to back on the last step's
+        state (state unchanged)."""
         ms = np.zeros(1, np.float32)
         _check(lib().pbf_time_stage(self._h, self.STAGES[stage], int(reps), _fp(ms)))
         return float(ms[0])

@@ -88,6 +88,8 @@ int reduce_blocks(const Tuning& T, int n);
 // predict + clamp + hash + count (reads the state, writes cell / local / count)
 void launch_predict_hash(hipStream_t st, const StepConsts& K, const Geometry& GE, const float4* pos, const float4* vel,
                          const StepBuffers& B);
+// an empty kernel of the particle grid's shape (launch-chain calibration)
+void launch_noop(hipStream_t st, int n);
 // exclusive scan of count into cstart; clears count for the next step
 void launch_scan(hipStream_t st, const StepConsts& K, const StepBuffers& B);
 // scatter into buckets, rank by particle id; the predicted state (recomputed from the old one),

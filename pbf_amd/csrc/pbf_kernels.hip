@@ -907,6 +907,11 @@ __global__ __launch_bounds__(kBlock) void k_import(int n, const float* __restric
     ids[s] = (uint32_t)s;
 }
 
+// Calibration of the launch chain: a kernel with the particle grid's shape (one workgroup of
+// kBlock per kBlock particles) that does nothing.  Replayed back to back in a graph, it gives
+// the cost of one dependent launch that the stages' short kernels cannot go below.
+__global__ __launch_bounds__(kBlock) void k_noop() {}
+
 // Plain streaming copy (counter calibration: reads and writes exactly `count` uint4).
 __global__ __launch_bounds__(kBlock) void k_copy_u128(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                                       size_t count) {
@@ -945,6 +950,10 @@ void launch_predict_hash(hipStream_t st, const StepConsts& K, const Geometry& GE
     if (K.n > 0)
         hipLaunchKernelGGL(k_predict_hash, dim3(nblk(K.n)), dim3(kBlock), 0, st, K, GE.geoms, GE.sid, pos, vel, B.cell,
                            B.local, B.count);
+}
+
+void launch_noop(hipStream_t st, int n) {
+    if (n > 0) hipLaunchKernelGGL(k_noop, dim3(nblk(n)), dim3(kBlock), 0, st);
 }
 
 void launch_scan(hipStream_t st, const StepConsts& K, const StepBuffers& B) {
