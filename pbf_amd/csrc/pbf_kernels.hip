@@ -309,18 +309,30 @@ __global__ __launch_bounds__(kBlock) void k_rank_reorder(StepConsts K, const Sce
     if (t >= K.n) return;
     const uint32_t e = bucket[t];
     const uint32_t id = bucket_id[t];
+    // the state gathers depend on e only: issued before the ranking's chain of loads
+    const float4 xo = pos_in[e];
+    float4 v = vel_in[e];
     const uint32_t c = cell[e];
     const SceneGeom G = geom_of(K, geoms, sid, (int)e);
     const bool inside = (int)c != G.coff + G.ncells;
     int d = t;
     if (inside) {
         const uint32_t st = cstart[c], en = cstart[c + 1];
+        // four members per round trip: the loads are clamped into [st, en) (always in the
+        // bucket) and the members past en are masked out of the count
         uint32_t rank = 0;
-        for (uint32_t q = st; q < en; ++q) rank += bucket_id[q] < id ? 1u : 0u;
+        for (uint32_t q = st; q < en; q += 4) {
+            const uint32_t last = en - 1;
+            const uint32_t a0 = bucket_id[q];
+            const uint32_t a1 = bucket_id[min(q + 1, last)];
+            const uint32_t a2 = bucket_id[min(q + 2, last)];
+            const uint32_t a3 = bucket_id[min(q + 3, last)];
+            rank += (a0 < id ? 1u : 0u) + (q + 1 < en && a1 < id ? 1u : 0u) + (q + 2 < en && a2 < id ? 1u : 0u) +
+                    (q + 3 < en && a3 < id ? 1u : 0u);
+        }
         d = (int)(st + rank);
     }
-    const float4 xo = pos_in[e];
-    float4 x = xo, v = vel_in[e];
+    float4 x = xo;
     predict(x, v, K, G);
     pos_out[d] = make_float4(x.x, x.y, x.z, 0.0f);
     vel_out[d] = v;
