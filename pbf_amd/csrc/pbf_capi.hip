@@ -178,7 +178,12 @@ void free_all(pbf_solver* s) {
     s->stream = nullptr;
 }
 
+// A captured step may still be running (steps only enqueue): the stream drains before the
+// executable graphs go.  Graphs are dropped on a parameter change, which recaptures anyway.
 void drop_graphs(pbf_solver* s) {
+    if (s->graphs.empty()) return;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (auto& g : s->graphs) (void)hipGraphExecDestroy(g.second);
     s->graphs.clear();
 }
@@ -334,6 +339,13 @@ int run_step(pbf_solver* s) {
         int rc = enqueue_step(s, nullptr);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(s->stream, &g);
+        // capture only recorded the step: restore the parity (also on the error paths below, so
+        // a failed capture leaves the solver as it was) and launch the graph for real below
+        s->pc = pc0;
+        s->vc = vc0;
+        s->ic = ic0;
+        s->lam_last = ll0;
+        s->steps_done = sd0;
         if (rc != PBF_OK) {
             if (g) (void)hipGraphDestroy(g);
             return rc;
@@ -344,12 +356,6 @@ int run_step(pbf_solver* s) {
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) return set_error(PBF_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
         found = s->graphs.emplace(key, ge).first;
-        // capture only recorded the step; restore the parity and launch it for real below
-        s->pc = pc0;
-        s->vc = vc0;
-        s->ic = ic0;
-        s->lam_last = ll0;
-        s->steps_done = sd0;
     }
     HIP_TRY(hipGraphLaunch(found->second, s->stream));
     // advance the parity exactly as enqueue_step does
