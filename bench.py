@@ -34,6 +34,8 @@ from pbf_amd import FLAG_GRAPH, DeviceBuffer, Solver, learned, lib, scenes  # no
 
 METRIC = "ms/step + particle-steps/s @262k; time to 0.5% density err, learned vs plain"
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak: 256 CUs x 4 SIMDs, one wave64 instruction per 4 cycles each, at 2.4 GHz
+VALU_PEAK = 256 * 4 * 2.4e9 / 4
 # Reference's published figure: 262144 particles, 0.012-0.015 s per cudaPBFSolve on an RTX 3090
 # (README.md:9); the faster end as particle-steps/s.
 PUBLISHED_PSPS = 262144 / 0.012
@@ -104,6 +106,22 @@ def measured_traffic(workload: str, stage: str):
             parts = line.split()
             if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch:" and parts[1] == kern:
                 return float(parts[2])
+    return None
+
+
+def measured_valu(workload: str, stage: str):
+    """wave64 VALU instructions per launch of the stage's kernel from the counter summary
+    profiles/valu_dam_break_262k_1m.md (lines 'valu_instr_per_launch: <workload> <kernel> <n>'),
+    or None."""
+    path = os.path.join(ROOT, "profiles", "valu_dam_break_262k_1m.md")
+    kern = STAGE_KERNEL.get(stage)
+    if kern is None or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 4 and parts[0] == "valu_instr_per_launch:" and parts[1] == workload and parts[2] == kern:
+                return float(parts[3])
     return None
 
 
@@ -236,6 +254,15 @@ def main():
     # hash, scan (2 launches up to 2048 tiles of 1024 cell counters, else 3), scatter, reorder
     sort_launches = 3 + (2 if ncells + 1 <= 2048 * 1024 else 3)
     floor_ms = s.time_stage("launch_floor", 50)  # an empty kernel of the same grid, replayed
+    # the constraint passes are bound by VALU issue: the dominant kernel's instruction rate
+    # against the nominal issue peak and against the probe's rate measured here
+    valu_instr = measured_valu(args.workload, dom)
+    probe_rate = Solver.VALU_PROBE_INSTR / (s.time_stage("valu_probe", 5) * 1e-3)
+    valu = None
+    if valu_instr is not None:
+        rate = valu_instr / (launch_ms * 1e-3)
+        valu = {"instr_per_launch": valu_instr, "achieved": rate, "peak": VALU_PEAK, "unit": "wave64 VALU instr/s",
+                "frac": rate / VALU_PEAK, "probe_rate": probe_rate, "frac_of_probe": rate / probe_rate}
 
     pos, vel = s.get_state()
     finite = bool(np.isfinite(pos).all() and np.isfinite(vel).all())
@@ -290,6 +317,8 @@ def main():
             "algorithmic_bytes_per_launch": dom_bytes,
             "avg_launch_ms": launch_ms,
             "avg_launch_ms_events_per_launch": avg_ms[dom],
+            "issue_bound": "valu",
+            "valu": valu,
         },
         "stages_ms": {k: round(v, 5) for k, v in per_step_ms.items()},
         "step_roofline_frac": step_bytes(n, ncells, iters) / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9),

@@ -362,13 +362,16 @@ class Solver:
         _check(lib().pbf_time_steps(self._h, int(nsteps), _fp(ms)))
         return float(ms[0])
 
-    STAGES = {"neighbors_lambda": 0, "lambda": 1, "delta_apply": 2, "sort_reorder": 3, "launch_floor": 4}
+    STAGES = {"neighbors_lambda": 0, "lambda": 1, "delta_apply": 2, "sort_reorder": 3, "launch_floor": 4,
+              "valu_probe": 5}
+    VALU_PROBE_INSTR = 2048 * 4 * 16 * 4096  # wave64 VALU instructions of one probe launch (loop body)
 
     def time_stage(self, stage: str, reps: int = 50) -> float:
         """ms per launch of one step stage ("neighbors_lambda", "lambda", "delta_apply",
-        "sort_reorder" = predict + hash, scan, scatter + rank + reorder, or "launch_floor" = an
-        empty kernel of the particle grid's shape), replayed back to back on the last step's
-        state (state unchanged)."""
+        "sort_reorder" = predict + hash, scan, scatter + rank + reorder, "launch_floor" = an
+        empty kernel of the particle grid's shape, or "valu_probe" = VALU_PROBE_INSTR multiply
+        and add instructions), replayed back to back on the last step's state (state
+        unchanged)."""
         ms = np.zeros(1, np.float32)
         _check(lib().pbf_time_stage(self._h, self.STAGES[stage], int(reps), _fp(ms)))
         return float(ms[0])

@@ -815,13 +815,14 @@ int pbf_profile_step(pbf_solver* s, float* ms, const char** names, int32_t max) 
 
 // Average duration of one launch of a step stage (0: neighbour build + lambda 1, 1: lambda,
 // 2: delta-p + apply, 3: predict + hash, scan, scatter + rank + reorder, 4: an empty kernel of
-// the particle grid's shape, the floor of one dependent launch), replayed reps times
+// the particle grid's shape, the floor of one dependent launch, 5: the VALU issue-rate probe),
+// replayed reps times
 // back to back on the state of the last step between two HIP events on the solver's stream.
 // The replays write only buffers that the next step rebuilds (cell tables, lists, lambda in
 // pos.w, the free position / velocity / id buffers, x_old; the scan clears the cell counters
 // again), so the state is unchanged.
 int pbf_time_stage(pbf_solver* s, int32_t stage, int32_t reps, float* ms) {
-    if (!s || !ms || reps < 1 || stage < 0 || stage > 4) return set_error(PBF_ERR_ARG, "bad argument");
+    if (!s || !ms || reps < 1 || stage < 0 || stage > 5) return set_error(PBF_ERR_ARG, "bad argument");
     if (s->steps_done < 1 || s->n == 0) return set_error(PBF_ERR_STATE, "run a step first");
     HIP_TRY(hipSetDevice(s->device));
     const StepConsts K = make_consts(s);
@@ -842,6 +843,7 @@ int pbf_time_stage(pbf_solver* s, int32_t stage, int32_t reps, float* ms) {
                                 B.vel[s->vc ^ 1], B.ids[s->ic ^ 1], nullptr);
         }
         if (stage == 4) launch_noop(s->stream, s->n);
+        if (stage == 5) launch_valu_probe(s->stream, reinterpret_cast<float*>(B.xp));
     }
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamEndCapture(s->stream, &g));

@@ -90,6 +90,11 @@ void launch_predict_hash(hipStream_t st, const StepConsts& K, const Geometry& GE
                          const StepBuffers& B);
 // an empty kernel of the particle grid's shape (launch-chain calibration)
 void launch_noop(hipStream_t st, int n);
+// VALU issue-rate calibration: kValuProbeBlocks workgroups of 256 (8 waves per SIMD on 1024
+// SIMDs), 16 * kValuProbeLinks wave64 VALU instructions per wavefront; sink is never written
+constexpr int kValuProbeBlocks = 2048;
+constexpr int kValuProbeLinks = 4096;
+void launch_valu_probe(hipStream_t st, float* sink);
 // exclusive scan of count into cstart; clears count for the next step
 void launch_scan(hipStream_t st, const StepConsts& K, const StepBuffers& B);
 // scatter into buckets, rank by particle id; the predicted state (recomputed from the old one),

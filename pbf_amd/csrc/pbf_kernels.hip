@@ -924,6 +924,22 @@ __global__ __launch_bounds__(kBlock) void k_import(int n, const float* __restric
 // the cost of one dependent launch that the stages' short kernels cannot go below.
 __global__ __launch_bounds__(kBlock) void k_noop() {}
 
+// Calibration of the VALU issue rate: 8 independent multiply-add chains per lane, each link 2
+// wave64 VALU instructions (no contraction, no packed math), kValuProbeLinks links: 65,536 VALU
+// instructions per wavefront plus a few for the set-up and the sum.  The sum is stored only if
+// it is NaN (never, for the host's constants), which keeps the chains without a store per lane.
+__global__ __launch_bounds__(kBlock) void k_valu_probe(float* __restrict__ sink, float m, float k) {
+    float x[8];
+    for (int i = 0; i < 8; ++i) x[i] = (float)(threadIdx.x + i);
+    for (int it = 0; it < kValuProbeLinks; ++it) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) x[i] = x[i] * m + k;
+    }
+    float s = 0.0f;
+    for (int i = 0; i < 8; ++i) s = s + x[i];
+    if (s != s) sink[0] = s;
+}
+
 // Plain streaming copy (counter calibration: reads and writes exactly `count` uint4).
 __global__ __launch_bounds__(kBlock) void k_copy_u128(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                                       size_t count) {
@@ -966,6 +982,10 @@ void launch_predict_hash(hipStream_t st, const StepConsts& K, const Geometry& GE
 
 void launch_noop(hipStream_t st, int n) {
     if (n > 0) hipLaunchKernelGGL(k_noop, dim3(nblk(n)), dim3(kBlock), 0, st);
+}
+
+void launch_valu_probe(hipStream_t st, float* sink) {
+    hipLaunchKernelGGL(k_valu_probe, dim3(kValuProbeBlocks), dim3(kBlock), 0, st, sink, 1.0000001f, 1e-7f);
 }
 
 void launch_scan(hipStream_t st, const StepConsts& K, const StepBuffers& B) {
