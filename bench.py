@@ -157,6 +157,51 @@ def copy_bandwidth(mb: int = 512, reps: int = 5) -> float:
     return float(2.0 * nbytes / (float(best) * 1e-3) / 1e9)
 
 
+def reference_cpu_step(scene, budget_s: float = 8.0):
+    """The reference's own CPU step, PBFSolver::solve (src/solver.cpp:60-157), built from its
+    sources into oracle/_ref/libpbfref.so, on the same scene: a bounded sample, or None when the
+    library is absent or the scene does not fit its fixed 64^3 grid.  Each call of the driver
+    constructs a solver, so the per-step time is (T(k steps) - T(1 step)) / (k - 1).  Its
+    physics is the CPU variant's (gravity -9.8, its own vorticity and jitter), and it prints
+    every particle's density in every iteration (sent to /dev/null here): timed as written."""
+    import ctypes as C
+
+    from oracle_lib import orc, ref, ref_available  # noqa: E402
+
+    h = float(scene.params["h"])
+    if not ref_available() or any(d > 64 for d in scene.grid_dims):
+        return None
+    L = ref()
+    fp = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+    L.ref_run.restype = C.c_int
+    L.ref_run.argtypes = [fp, fp, fp, fp, C.c_int, C.c_int]
+    p = scene.params
+    params = np.array([h, p["restRho"], p["epsilon"], p["k"], p["n"], p["deltaq"], p["radius"], p["c"], p["dt"],
+                       p["iterations"], p["vorticityConfinementEpilon"]], np.float32)
+    box = np.ascontiguousarray(scene.box, np.float32)
+
+    def run(k):
+        pos = np.ascontiguousarray(scene.pos, np.float32).copy()
+        vel = np.ascontiguousarray(scene.vel, np.float32).copy()
+        t0 = time.perf_counter()
+        L.ref_run(params, box, pos, vel, scene.n, k)
+        return time.perf_counter() - t0
+
+    t1 = run(1)
+    k = int(max(3, min(20, budget_s / max(t1, 1e-3))))
+    tk = run(k)
+    per = max(tk - t1, 1e-9) / (k - 1)
+    return {
+        "value": scene.n / per,
+        "unit": "particle-steps/s",
+        "cores": int(orc().orc_max_threads()),
+        "kind": "reference",
+        "sample": f"{k} steps (and 1 for the construction) of {scene.name} ({scene.n} particles), {t1 + tk:.1f} s",
+        "ms_per_step": 1000.0 * per,
+        "note": "src/solver.cpp PBFSolver::solve as written: its CPU physics variant, per-particle printing",
+    }
+
+
 def cpu_baseline(scene, budget_s: float = 12.0):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Oracle, orc  # noqa: E402
@@ -180,6 +225,7 @@ def cpu_baseline(scene, budget_s: float = 12.0):
         "kind": "port",
         "sample": f"{steps} steps of {scene.name} ({scene.n} particles) after 1 warm-up step, {el:.1f} s",
         "ms_per_step": 1000.0 * el / steps,
+        "reference_solver": reference_cpu_step(scene),
     }
 
 
