@@ -1,6 +1,7 @@
 """Learned warm start: dataset from the plain solver, on-device training, evaluation.
 
     python -m pbf_amd.learned [--quick] [--out results.json]
+    python -m pbf_amd.learned --trace-steps 40      # train, then learned and plain steps to profile
 
 1. Record: for each training seed, run the dam break with its own (4-iteration) steps; from each
    state, a plain step of ``record_iters`` (8) iterations run from a device snapshot records, for
@@ -182,11 +183,33 @@ def evaluate(m, seeds, warmup=60, nstates=6, gap=8, n_side=(64, 64, 64), box=(0,
     return rows, summary
 
 
+def trace_steps(m, sc, passes, steps, warmup=60):
+    """For a kernel trace: `steps` graph-replayed learned steps of `passes` passes, then `steps`
+    plain steps of as many iterations, each from the scene after `warmup` steps."""
+    s = Solver.from_scene(sc, flags=FLAG_GRAPH)
+    s.advance(warmup)
+    s.attach_mlp(m)
+    s.set_learned(passes)
+    s.advance(steps)
+    s.synchronize()
+    kept = bool(s.last_guess_info()[2])
+    s.close()
+    s = Solver.from_scene(sc, flags=FLAG_GRAPH, iterations=passes)
+    s.advance(warmup + steps)
+    s.synchronize()
+    s.close()
+    return kept
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--quick", action="store_true", help="small scenes and short runs")
     ap.add_argument("--out", default=None)
     ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--trace-steps", type=int, default=0,
+                    help="after training, run this many learned steps of --trace-passes passes and as many "
+                         "plain steps (on the first held-out seed) instead of the evaluation")
+    ap.add_argument("--trace-passes", type=int, default=7)
     args = ap.parse_args()
     if args.quick:
         n_side, box = (32, 32, 32), (0, 0, 0, 3.2, 2.0, 1.6)
@@ -201,6 +224,11 @@ def main():
     print(f"dataset: {nvalid} samples; target mean {ym:.4g} std {ys:.4g}", flush=True)
     m, losses = train(ds, nvalid, mean, inv, ym, ys, steps=args.train_steps)
     ds_close()
+    if args.trace_steps > 0:
+        sc = scenes.dam_break(n_side, box, seed=eval_seeds[0])
+        kept = trace_steps(m, sc, args.trace_passes, args.trace_steps, warm)
+        print(json.dumps({"trace_steps": args.trace_steps, "passes": args.trace_passes, "last_guess_kept": kept}), flush=True)
+        return
     rows_, summ = evaluate(m, list(eval_seeds), warm, n_side=n_side, box=box)
     out = dict(samples=nvalid, loss_start=float(losses[:20].mean()), loss_end=float(losses[-100:].mean()), summary=summ, states=rows_)
     print(json.dumps(summ), flush=True)
