@@ -16,6 +16,8 @@
 // workgroup, and summed over workgroups in a fixed order (deterministic).
 
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <stdint.h>
 
 #include "pbf_kernels.h"
@@ -361,7 +363,11 @@ int mlp_grad_blocks(int nb) {
 void launch_mlp_infer(hipStream_t st, const MlpModel* M, const float* feat, int n, float4* pos, float* y_out,
                       float lam_scale) {
     if (n <= 0) return;
-    int blocks = (n + 63) / 64;
+    // The kernel hoists the weight fragments, biases and output weights of its wavefront into
+    // registers once (about 100 VGPRs, 3 waves per SIMD); a grid of about 6 workgroups per CU,
+    // each looping over several 64-sample chunks, reuses them instead of reloading them per
+    // chunk (measured: 5 us less per learned step at 262k than one workgroup per chunk).
+    int blocks = std::min((n + 63) / 64, 1536);
     hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(kB), 0, st, M, feat, n, pos, y_out, lam_scale);
 }
 
