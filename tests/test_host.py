@@ -103,3 +103,22 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([_sys.executable, _os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_bench_reads_the_committed_counter_summaries():
+    """bench.py's roofline takes the dominant kernel's measured traffic and VALU instruction count
+    from the committed rocprofv3 summaries; both stay machine-readable for the kernels it can name,
+    at 262k and (VALU) 1M, and an unknown kernel or workload gives None."""
+    import importlib.util
+    import os as _os
+
+    root = _os.path.dirname(_os.path.dirname(_os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench_under_test", _os.path.join(root, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    for stage in ("delta_apply", "lambda", "curl", "vort_apply"):
+        assert bench.measured_traffic("dam_break_262k", stage) > 1e6
+        for wl in ("dam_break_262k", "double_dam_break_1m"):
+            assert bench.measured_valu(wl, stage) > 1e6
+    assert bench.measured_valu("dam_break_262k", "no_such_stage") is None
+    assert bench.measured_valu("no_such_workload", "delta_apply") is None
