@@ -89,13 +89,14 @@ int pbf_create_batch(const pbf_params* para, int32_t nscenes, const int32_t* cou
 
 /* Launch shape of the neighbour kernels: block 64/128/256/512, grid > 0 = persistent grid of
  * that many workgroups (0 = one per block), xcd_map = contiguous particle chunks per XCD.
- * Results do not depend on it. */
+ * Results do not depend on it.  No counterpart in the reference, which launches every kernel
+ * with one fixed shape (kernels/cudaPBF.cu:54-56). */
 int pbf_set_tuning(pbf_solver* s, int32_t block, int32_t grid, int32_t xcd_map);
 int pbf_get_tuning(pbf_solver* s, int32_t out[3]);
 
 /* Autotuner: times every launch shape over `steps` graph-replayed steps, keeps the fastest and
  * restores the state (tuning does not advance the simulation).  best[3] = block, grid,
- * xcd_map; best_ms = time per step.  Either may be NULL. */
+ * xcd_map; best_ms = time per step.  Either may be NULL.  No counterpart in the reference. */
 int pbf_autotune(pbf_solver* s, int32_t steps, int32_t* best, float* best_ms);
 
 /* Replaces cudaPBFSolve (kernels/cudaPBF.h:12, kernels/cudaPBF.cu:50-114): one solver step
@@ -104,16 +105,19 @@ int pbf_autotune(pbf_solver* s, int32_t steps, int32_t* best, float* best_ms);
 int pbf_solve(pbf_solver* s, const pbf_params* para);
 
 /* One step with the solver's current parameters and time step dt (para.dt = dt,
- * para.invDt = 1/dt), as cudaPBFSolve called with an updated para.dt. */
+ * para.invDt = 1/dt), as cudaPBFSolve (kernels/cudaPBF.cu:50) called with an updated para.dt. */
 int pbf_step(pbf_solver* s, float dt);
 
-/* nsteps steps with the current parameters. */
+/* nsteps steps with the current parameters: the app's frame loop of cudaPBFSolve calls
+ * (src/renderapp.cpp:497-502) without its per-frame host timing. */
 int pbf_advance(pbf_solver* s, int32_t nsteps);
 
-/* Host copy of the state in the caller's particle order (either pointer may be NULL). */
+/* Host copy of the state in the caller's particle order (either pointer may be NULL): what the
+ * app reads from SolverData.position / .velocity (include/solverBase.h:45-47). */
 int pbf_get_state(pbf_solver* s, float* pos_xyz, float* vel_xyz);
 
-/* Replace the state (caller's particle order; vel_xyz NULL = zero). */
+/* Replace the state (caller's particle order; vel_xyz NULL = zero): the upload of
+ * initCUDASolverDataBuffer (kernels/cudaPBF.cu:4-30) into an existing solver. */
 int pbf_set_state(pbf_solver* s, const float* pos_xyz, const float* vel_xyz);
 
 /* Write positions (and velocities if non-NULL) in the caller's particle order into DEVICE
@@ -124,17 +128,20 @@ int pbf_export_device(pbf_solver* s, float* dev_pos_xyz, float* dev_vel_xyz);
 /* Replaces freeCUDASolverDataBuffer (kernels/cudaPBF.h:11, kernels/cudaPBF.cu:32-48). */
 int pbf_destroy(pbf_solver* s);
 
-/* Text of the calling thread's last error. */
+/* Text of the calling thread's last error (the reference's CHECK_CUDA prints and exits instead,
+ * kernels/cudaPBFSolver.cuh:5-10). */
 const char* pbf_last_error(void);
 
-/* Current parameters of the solver. */
+/* Current parameters of the solver.  No counterpart in the reference (the app keeps para). */
 int pbf_get_params(pbf_solver* s, pbf_params* out);
 
-/* Block until the solver's queued work is done. */
+/* Block until the solver's queued work is done (the reference synchronises the device after
+ * every launch, kernels/cudaPBF.cu:62-113). */
 int pbf_synchronize(pbf_solver* s);
 
 /* Device view of the cell-sorted state after the last step: n float4 positions (x, y, z, |curl|
- * of the step), n float4 velocities (w = 0), uint32 particle ids per slot. */
+ * of the step), n float4 velocities (w = 0), uint32 particle ids per slot -- the device pointers
+ * the reference's SolverData exposes (include/solverBase.h:45-47), in slot order. */
 typedef struct pbf_device_view {
     const void* pos4;
     const void* vel4;
@@ -144,7 +151,10 @@ typedef struct pbf_device_view {
 } pbf_device_view;
 int pbf_device_state(pbf_solver* s, pbf_device_view* out);
 
-/* ---- diagnostics of the last step (need PBF_FLAG_DEBUG at create / solve time) ----------- */
+/* ---- diagnostics of the last step (need PBF_FLAG_DEBUG at create / solve time) -----------
+ * The intermediates the reference keeps in SolverData (include/solverBase.h:45-63: grid,
+ * gridParticlesCnt, gridParticlesID, neighbors, lambda, particlesCurl), read back by particle
+ * id; the reference has no read-back functions of its own. */
 
 /* gid[n] by particle id (-1 = outside the grid), count[ncells], cell_start[ncells+1]
  * (cell_start[ncells] = particles inside the grid), sorted_ids[n] (particle ids in cell
@@ -166,7 +176,8 @@ int pbf_get_fields(pbf_solver* s, float* lam_first, float* lam_last, float* err,
  * out[2] = max err.  Needs PBF_FLAG_DENSITY_STATS or PBF_FLAG_DEBUG. */
 int pbf_density_stats(pbf_solver* s, float out[3]);
 
-/* ---- measurement -------------------------------------------------------------------------- */
+/* ---- measurement (no counterpart in the reference, which times cudaPBFSolve with the host
+ * clock, src/renderapp.cpp:498-501) ---------------------------------------------------------- */
 
 /* Runs nsteps steps and returns the elapsed time measured with HIP events recorded on the
  * solver's stream around them. */
@@ -252,26 +263,28 @@ int pbf_set_learned(pbf_solver* s, int32_t passes);
  * reorder, 4: an empty kernel of the particle grid's shape, i.e. the cost of one dependent
  * launch, 5: a VALU issue-rate probe of 2048 x 256 threads issuing 65,536 wave64 VALU
  * instructions per wavefront), replayed reps times back to back on the state of the last step,
- * between HIP events
- * on the solver's stream.  The state is unchanged (the replays write only what the next step
- * rebuilds); the diagnostics of the last step (pbf_get_grid, pbf_get_neighbors, pbf_get_fields)
- * no longer describe it afterwards. */
+ * between HIP events on the solver's stream.  The state is unchanged (the replays write only
+ * what the next step rebuilds); the diagnostics of the last step (pbf_get_grid,
+ * pbf_get_neighbors, pbf_get_fields) no longer describe it afterwards.  No counterpart in the
+ * reference. */
 int pbf_time_stage(pbf_solver* s, int32_t stage, int32_t reps, float* ms);
 
 /* save != 0: copy the current state (positions, velocities, particle order) into one device-side
- * snapshot of the solver; save == 0: restore the state from it.  Stream-ordered, no host copies. */
+ * snapshot of the solver; save == 0: restore the state from it.  Stream-ordered, no host copies.
+ * No counterpart in the reference. */
 int pbf_snapshot(pbf_solver* s, int32_t save);
 
-/* Device memory helpers for callers without an allocator (datasets, export targets). */
+/* Device memory helpers for callers without an allocator (datasets, export targets).  No
+ * counterpart in the reference. */
 int pbf_device_alloc(uint64_t bytes, void** out);
 int pbf_device_free(void* p);
 int pbf_memcpy_to_host(void* dst, const void* src, uint64_t bytes);
 int pbf_memcpy_to_device(void* dst, const void* src, uint64_t bytes);
 /* Streaming copy kernel of `bytes` (multiple of 16) on the null stream, timed with HIP events:
- * a known-traffic reference for hardware counter calibration. */
+ * a known-traffic calibration for hardware counters.  No counterpart in the reference. */
 int pbf_copy_kernel(void* dst, const void* src, uint64_t bytes, float* ms);
 
-/* Library build string. */
+/* Library build string.  No counterpart in the reference. */
 const char* pbf_version(void);
 
 #ifdef __cplusplus
