@@ -570,8 +570,7 @@ __global__ __launch_bounds__(BS) void k_search(StepConsts K, const SceneGeom* __
 // With L.feat, C at the predicted positions goes to feature slot 10; with L.resid the workgroup
 // sums of max(C, 0) and |C| at the predicted positions.
 template <int BS>
-__global__ __launch_bounds__(BS) void k_lambda_first(StepConsts K, const SceneGeom* __restrict__ geoms,
-                                                     const uint8_t* __restrict__ sid, float4* __restrict__ pos,
+__global__ __launch_bounds__(BS) void k_lambda_first(StepConsts K, float4* __restrict__ pos,
                                                      const uint32_t* __restrict__ nbr, const uint8_t* __restrict__ ncnt,
                                                      float* __restrict__ lam1, float* __restrict__ dbg_lam,
                                                      LearnArgs L, int xcd_map) {
@@ -1030,8 +1029,8 @@ void launch_neighbors_lambda(hipStream_t st, const StepConsts& K, const Geometry
         PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_search<BS, false>), dim3(ngrid_search(T, K.n)), dim3(BS), 0, st, K,
                                               GE.geoms, GE.sid, pos, B.cstart, B.nbr, B.ncnt, T.xcd_map));
     }
-    PBF_DISPATCH_BS(T, hipLaunchKernelGGL(k_lambda_first<BS>, dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
-                                          GE.sid, pos, B.nbr, B.ncnt, B.lam1, dbg_lam, L, T.xcd_map));
+    PBF_DISPATCH_BS(T, hipLaunchKernelGGL(k_lambda_first<BS>, dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, B.nbr,
+                                          B.ncnt, B.lam1, dbg_lam, L, T.xcd_map));
 }
 
 void launch_lambda(hipStream_t st, const StepConsts& K, const Geometry& GE, const Tuning& T, float4* pos,
