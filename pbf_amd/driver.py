@@ -19,7 +19,7 @@ import time
 
 import numpy as np
 
-from . import Solver, SolverParameter, statefile
+from . import Solver, statefile
 from .scenes import Scene, uniform24
 
 APP_PARAMS = dict(

@@ -1,6 +1,5 @@
 """Host-side pieces without a GPU: scene generators, the reference app's scenes, the state
 file, and the batch workload."""
-import math
 
 import numpy as np
 import pytest

@@ -336,8 +336,6 @@ def test_oracle_against_reference_golden_vectors():
 @pytest.mark.skipif(not ref_available(), reason="reference build oracle/_ref not present")
 def test_golden_vectors_are_reproducible():
     """The committed vectors are what the live reference build produces."""
-    import ctypes as C  # noqa: F401
-
     g = np.load(os.path.join(GOLDEN, "ref_probe_3k.npz"))
     n = len(g["pos"])
     cells = ref().ref_grid_cells()
