@@ -37,6 +37,25 @@ def test_forward_matches_oracle():
     assert np.median(np.abs(y - ref)) <= 2e-3 * scale
 
 
+def test_forward_many_chunks_per_workgroup():
+    """Past 1536 workgroups x 64 samples each inference workgroup loops over several chunks with
+    its weights held in registers: 200,000 samples in one call give the same bits as the same
+    samples in calls of at most 60,000 (one chunk per workgroup), and the samples past the first
+    98,304 agree with the oracle."""
+    print("[case] mlp forward, 200k samples", flush=True)
+    m, p = identity_net(5)
+    rng = np.random.default_rng(6)
+    x = np.clip(rng.standard_normal((200_000, 16)), -7.5, 7.5).astype(np.float32)
+    y = m.forward(x)
+    pieces = np.concatenate([m.forward(x[i:i + 60_000]) for i in range(0, len(x), 60_000)])
+    assert np.array_equal(y.view(np.uint32), pieces.view(np.uint32))
+    tail = slice(150_000, 150_300)
+    ref = mlp_forward(p, x[tail], bf16=True)
+    scale = np.abs(ref).max()
+    assert np.abs(y[tail] - ref).max() <= 2e-2 * scale
+    assert np.median(np.abs(y[tail] - ref)) <= 2e-3 * scale
+
+
 def test_gradients_match_oracle():
     print("[case] mlp gradients", flush=True)
     m, p = identity_net(5)
