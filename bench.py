@@ -273,6 +273,7 @@ def main():
     tuning, tune_ms = s.get_tuning(), None
     if not args.no_autotune:
         tuning, tune_ms = s.autotune(10)
+    row_order = ("z-y", "morton-yz")[s.get_ordering()]
     ms_total = s.time_steps(args.steps)
     ms_step = ms_total / max(args.steps, 1)
     value = n / (ms_step * 1e-3)
@@ -350,7 +351,8 @@ def main():
             "dt": scene.params["dt"],
             "graph": not args.no_graph,
             "parallelism": "single GPU",
-            "tuning": {"block": tuning[0], "grid": tuning[1], "xcd_map": tuning[2], "autotuned": not args.no_autotune},
+            "tuning": {"block": tuning[0], "grid": tuning[1], "xcd_map": tuning[2], "row_order": row_order,
+                       "autotuned": not args.no_autotune},
         },
         "roofline": {
             "bound": "hbm",

@@ -94,9 +94,20 @@ int pbf_create_batch(const pbf_params* para, int32_t nscenes, const int32_t* cou
 int pbf_set_tuning(pbf_solver* s, int32_t block, int32_t grid, int32_t xcd_map);
 int pbf_get_tuning(pbf_solver* s, int32_t out[3]);
 
-/* Autotuner: times every launch shape over `steps` graph-replayed steps, keeps the fastest and
- * restores the state (tuning does not advance the simulation).  best[3] = block, grid,
- * xcd_map; best_ms = time per step.  Either may be NULL.  No counterpart in the reference. */
+/* Order of the cell rows (the grid_dims[0] cells of one (y, z)) in the cell table, which is also
+ * the particles' order in memory: 0 = z, then y (the reference's computeGid order,
+ * kernels/cudaPBFSolver.cuh:156-160); 1 = a Morton curve over (y, z), per scene.  Cells stay
+ * x-fastest inside a row, and the neighbour lists keep the reference's z / y / x order either
+ * way, so results do not depend on it (pbf_get_grid always reports the reference's cell
+ * order).  No counterpart in the reference. */
+int pbf_set_ordering(pbf_solver* s, int32_t order);
+int pbf_get_ordering(pbf_solver* s, int32_t* out);
+
+/* Autotuner: times every launch shape over `steps` graph-replayed steps, keeps the fastest, then
+ * times that shape with the other row order (pbf_set_ordering) and keeps the faster; it restores
+ * the state (tuning does not advance the simulation).  best[3] = block, grid, xcd_map; best_ms =
+ * time per step; the order chosen is pbf_get_ordering's.  Either may be NULL.  No counterpart in
+ * the reference. */
 int pbf_autotune(pbf_solver* s, int32_t steps, int32_t* best, float* best_ms);
 
 /* Replaces cudaPBFSolve (kernels/cudaPBF.h:12, kernels/cudaPBF.cu:50-114): one solver step

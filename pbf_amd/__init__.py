@@ -171,6 +171,8 @@ def lib() -> C.CDLL:
         "pbf_set_tuning": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32]),
         "pbf_get_tuning": (C.c_int, [_VP, _I]),
         "pbf_autotune": (C.c_int, [_VP, C.c_int32, _I, _F]),
+        "pbf_set_ordering": (C.c_int, [_VP, C.c_int32]),
+        "pbf_get_ordering": (C.c_int, [_VP, _I]),
         "pbf_device_alloc": (C.c_int, [C.c_uint64, C.POINTER(_VP)]),
         "pbf_device_free": (C.c_int, [_VP]),
         "pbf_memcpy_to_host": (C.c_int, [_VP, _VP, C.c_uint64]),
@@ -257,8 +259,19 @@ class Solver:
         _check(lib().pbf_get_tuning(self._h, _ip(out)))
         return int(out[0]), int(out[1]), bool(out[2])
 
+    def set_ordering(self, order: int):
+        """Cell-row order in memory: 0 = z, then y; 1 = Morton curve over (y, z).  Results do not
+        depend on it."""
+        _check(lib().pbf_set_ordering(self._h, int(order)))
+
+    def get_ordering(self) -> int:
+        out = np.zeros(1, np.int32)
+        _check(lib().pbf_get_ordering(self._h, _ip(out)))
+        return int(out[0])
+
     def autotune(self, steps: int = 10):
-        """Picks the fastest launch shape (state unchanged); returns ((block, grid, xcd_map), ms/step)."""
+        """Picks the fastest launch shape, then the faster row order for it (state unchanged);
+        returns ((block, grid, xcd_map), ms/step); the order is get_ordering()'s."""
         best = np.zeros(3, np.int32)
         ms = np.zeros(1, np.float32)
         _check(lib().pbf_autotune(self._h, int(steps), _ip(best), _fp(ms)))

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
@@ -95,6 +96,11 @@ struct pbf_solver {
     // graph replay, one graph per (step kind, state parity)
     std::map<int, hipGraphExec_t> graphs;
     std::vector<void*> allocs;
+    // cell-row order (pbf_set_ordering): 0 = z, then y; 1 = a Morton curve over (y, z) per scene
+    std::vector<SceneGeom> scenes;  // host copy of each scene's grid (rowoff, dims)
+    int order = 0;
+    int* rowmap = nullptr;          // device: the Morton rows of every scene (built on first use)
+    std::vector<int> rowmap_host;   // the same on the host (pbf_get_grid)
 };
 
 namespace {
@@ -140,6 +146,7 @@ StepConsts make_consts(const pbf_solver* s) {
     K.n = s->n;
     K.nstride = s->nstride;
     K.nbuckets = s->nbuckets;
+    K.rowmap = s->order ? s->rowmap : nullptr;
     return K;
 }
 
@@ -400,6 +407,14 @@ int create_solver(const pbf_params* para, const float* grid_space, const float* 
     s->nstride = ((n + 63) / 64) * 64;
     if (s->nstride == 0) s->nstride = 64;
     s->cap = para->max_neighbors;
+    if (geoms) {
+        s->scenes = *geoms;
+    } else {
+        SceneGeom G{};
+        for (int a = 0; a < 3; ++a) G.dims[a] = para->grid_dims[a];
+        G.ncells = s->ncells;
+        s->scenes.push_back(G);
+    }
     (void)hipGetDevice(&s->device);
     auto fail = [&](int code) {
         free_all(s);
@@ -494,7 +509,7 @@ int pbf_create_batch(const pbf_params* para, int32_t nscenes, const int32_t* cou
     if (nscenes < 1 || nscenes > 255 || !counts || !grid_space || !boundary || !grid_dims)
         return set_error(PBF_ERR_ARG, "bad scene description (1..255 scenes)");
     std::vector<SceneGeom> geoms(nscenes);
-    int64_t n = 0, nb = 0;
+    int64_t n = 0, nb = 0, rows = 0;
     for (int k = 0; k < nscenes; ++k) {
         if (counts[k] < 0) return set_error(PBF_ERR_ARG, "negative scene count");
         SceneGeom& G = geoms[k];
@@ -510,6 +525,8 @@ int pbf_create_batch(const pbf_params* para, int32_t nscenes, const int32_t* cou
         }
         G.coff = (int)nb;
         G.ncells = (int)cells;
+        G.rowoff = (int)rows;
+        rows += (int64_t)G.dims[1] * G.dims[2];
         nb += cells + 1;
         n += counts[k];
         if (nb > ((int64_t)1 << 28) || n >= kMaxSlots) return set_error(PBF_ERR_ARG, "batch too large");
@@ -531,6 +548,52 @@ int pbf_set_tuning(pbf_solver* s, int32_t block, int32_t grid, int32_t xcd_map) 
     s->tune.grid = grid;
     s->tune.xcd_map = xcd_map ? 1 : 0;
     drop_graphs(s);
+    return PBF_OK;
+}
+
+namespace {
+// Morton key of row (y, z): the bits of y and z interleaved (y in the even bits).
+uint32_t morton2(uint32_t y, uint32_t z) {
+    uint32_t k = 0;
+    for (int b = 0; b < 16; ++b) k |= ((y >> b) & 1u) << (2 * b) | ((z >> b) & 1u) << (2 * b + 1);
+    return k;
+}
+}  // namespace
+
+int pbf_set_ordering(pbf_solver* s, int32_t order) {
+    if (!s) return set_error(PBF_ERR_ARG, "solver is NULL");
+    if (order != 0 && order != 1) return set_error(PBF_ERR_ARG, "order must be 0 (z, y rows) or 1 (Morton rows)");
+    if (order == 1 && !s->rowmap) {
+        size_t total = 0;
+        for (const SceneGeom& G : s->scenes) total += (size_t)G.dims[1] * G.dims[2];
+        std::vector<int> map(total);
+        for (const SceneGeom& G : s->scenes) {
+            const int ny = G.dims[1], nz = G.dims[2];
+            std::vector<int> r(ny * nz);
+            for (int i = 0; i < ny * nz; ++i) r[i] = i;
+            std::sort(r.begin(), r.end(), [&](int a, int b) {
+                return morton2((uint32_t)(a % ny), (uint32_t)(a / ny)) < morton2((uint32_t)(b % ny), (uint32_t)(b / ny));
+            });
+            for (int rank = 0; rank < ny * nz; ++rank) map[(size_t)G.rowoff + r[rank]] = rank;
+        }
+        HIP_TRY(hipSetDevice(s->device));
+        int* d = nullptr;
+        int rc = dalloc(s, &d, total);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpy(d, map.data(), sizeof(int) * total, hipMemcpyHostToDevice));
+        s->rowmap = d;
+        s->rowmap_host = std::move(map);
+    }
+    if (order != s->order) {
+        s->order = order;
+        drop_graphs(s);
+    }
+    return PBF_OK;
+}
+
+int pbf_get_ordering(pbf_solver* s, int32_t* out) {
+    if (!s || !out) return set_error(PBF_ERR_ARG, "NULL argument");
+    *out = s->order;
     return PBF_OK;
 }
 
@@ -669,12 +732,34 @@ int pbf_get_grid(pbf_solver* s, int32_t* gid, int32_t* count, int32_t* cell_star
     std::vector<uint32_t> cs(s->ncells + 1);
     HIP_TRY(hipMemcpy(cs.data(), s->B.cstart, sizeof(uint32_t) * cs.size(), hipMemcpyDeviceToHost));
     if (gid && s->n) HIP_TRY(hipMemcpy(gid, s->dbg_gid, sizeof(int32_t) * s->n, hipMemcpyDeviceToHost));
-    if (cell_start)
-        for (int c = 0; c <= s->ncells; ++c) cell_start[c] = (int32_t)cs[c];
-    if (count)
-        for (int c = 0; c < s->ncells; ++c) count[c] = (int32_t)(cs[c + 1] - cs[c]);
-    if (sorted_ids && s->n)
-        HIP_TRY(hipMemcpy(sorted_ids, s->B.ids[s->ic], sizeof(int32_t) * s->n, hipMemcpyDeviceToHost));
+    if (!s->order) {
+        if (cell_start)
+            for (int c = 0; c <= s->ncells; ++c) cell_start[c] = (int32_t)cs[c];
+        if (count)
+            for (int c = 0; c < s->ncells; ++c) count[c] = (int32_t)(cs[c + 1] - cs[c]);
+        if (sorted_ids && s->n)
+            HIP_TRY(hipMemcpy(sorted_ids, s->B.ids[s->ic], sizeof(int32_t) * s->n, hipMemcpyDeviceToHost));
+        return (int)cs[s->ncells];
+    }
+    // Morton rows: the table in the reference's cell order (x-fastest, then y, then z), the
+    // members of each cell as they lie in its bucket, then the out-of-grid particles
+    std::vector<uint32_t> ids(s->n);
+    if (s->n) HIP_TRY(hipMemcpy(ids.data(), s->B.ids[s->ic], sizeof(uint32_t) * s->n, hipMemcpyDeviceToHost));
+    const int dx = s->P.grid_dims[0];
+    uint32_t run = 0;
+    size_t k = 0;
+    for (int c = 0; c < s->ncells; ++c) {
+        const int b = s->rowmap_host[c / dx] * dx + c % dx;
+        const uint32_t len = cs[b + 1] - cs[b];
+        if (cell_start) cell_start[c] = (int32_t)run;
+        if (count) count[c] = (int32_t)len;
+        if (sorted_ids)
+            for (uint32_t j = cs[b]; j < cs[b + 1]; ++j) sorted_ids[k++] = (int32_t)ids[j];
+        run += len;
+    }
+    if (cell_start) cell_start[s->ncells] = (int32_t)run;
+    if (sorted_ids)
+        for (uint32_t j = cs[s->ncells]; j < (uint32_t)s->n; ++j) sorted_ids[k++] = (int32_t)ids[j];
     return (int)cs[s->ncells];
 }
 
@@ -905,29 +990,47 @@ int pbf_autotune(pbf_solver* s, int32_t steps, int32_t* best, float* best_ms) {
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
+    // ms per step of the current shape and order: 2 steps to capture and warm up, then `steps`
+    auto time_config = [&](float& ms) -> int {
+        drop_graphs(s);
+        for (int w = 0; w < 2; ++w)
+            if (int r = run_step(s)) return r;
+        if (hipEventRecord(e0, s->stream) != hipSuccess) return set_error(PBF_ERR_HIP, "event");
+        for (int k = 0; k < steps; ++k)
+            if (int r = run_step(s)) return r;
+        if (hipEventRecord(e1, s->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+            return set_error(PBF_ERR_HIP, "event");
+        ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= (float)steps;
+        return PBF_OK;
+    };
     for (int bi = 0; bi < 4 && rc == PBF_OK; ++bi)
         for (int gi = 0; gi < 4 && rc == PBF_OK; ++gi)
             for (int xm = 0; xm < 2 && rc == PBF_OK; ++xm) {
                 if (grids[gi] > 0 && (int)((n + blocks[bi] - 1) / blocks[bi]) <= grids[gi]) continue;
                 s->tune = Tuning{blocks[bi], grids[gi], xm};
-                drop_graphs(s);
-                for (int w = 0; w < 2 && rc == PBF_OK; ++w) rc = run_step(s);  // capture + warm up
-                if (rc) break;
-                if (hipEventRecord(e0, s->stream) != hipSuccess) { rc = set_error(PBF_ERR_HIP, "event"); break; }
-                for (int k = 0; k < steps && rc == PBF_OK; ++k) rc = run_step(s);
-                if (rc) break;
-                if (hipEventRecord(e1, s->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
-                    rc = set_error(PBF_ERR_HIP, "event");
-                    break;
-                }
                 float ms = 0.0f;
-                (void)hipEventElapsedTime(&ms, e0, e1);
-                ms /= (float)steps;
-                if (ms < bestMs) {
+                rc = time_config(ms);
+                if (rc == PBF_OK && ms < bestMs) {
                     bestMs = ms;
                     bestT = s->tune;
                 }
             }
+    // the best shape with the other row order
+    const int order0 = s->order;
+    int bestOrder = order0;
+    if (rc == PBF_OK) {
+        s->tune = bestT;
+        rc = pbf_set_ordering(s, 1 - order0);
+        float ms = 0.0f;
+        if (rc == PBF_OK) rc = time_config(ms);
+        if (rc == PBF_OK && ms < bestMs) {
+            bestMs = ms;
+            bestOrder = 1 - order0;
+        }
+    }
+    s->order = rc == PBF_OK ? bestOrder : order0;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     // restore the state the tuning started from

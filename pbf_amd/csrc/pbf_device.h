@@ -22,6 +22,7 @@ struct SceneGeom {
     int dims[3];
     int coff;
     int ncells;
+    int rowoff;  // first entry of the scene's row order in StepConsts::rowmap
 };
 
 // Everything a step kernel needs, passed by value as one kernel argument.
@@ -38,6 +39,11 @@ struct StepConsts {
     int n;       // particles
     int nstride; // row stride of the neighbour table (>= n, multiple of 64)
     int nbuckets;  // cell counters of the whole step (ncells + 1, or the sum over a batch)
+    // Order of the cell rows (the dims[0] cells of one (y, z)) in the cell table: NULL = z, then
+    // y (x-fastest throughout); else rowmap[rowoff + z * dims[1] + y] is the row's place.  Cells
+    // stay x-fastest inside a row either way, and the neighbour lists and sums do not depend on
+    // the order of the rows.
+    const int* rowmap;
 };
 
 // The geometry of a single-scene solver, from its constants.
@@ -51,6 +57,7 @@ __device__ __forceinline__ SceneGeom geom_single(const StepConsts& K) {
     }
     G.coff = 0;
     G.ncells = K.ncells;
+    G.rowoff = 0;
     return G;
 }
 
@@ -160,12 +167,29 @@ __device__ __forceinline__ int dim_of(float p, float pmin, float invH, int dim) 
 
 // computeGid, cuh:172-181, offset by the scene's first cell; out-of-grid particles go to the
 // scene's bucket coff + ncells.
+// Place of row (y, z), 0 <= y < dims[1], 0 <= z < dims[2], in the scene's cell table.
+__device__ __forceinline__ int row_of(int y, int z, const StepConsts& K, const SceneGeom& G) {
+    const int r = z * G.dims[1] + y;
+    return K.rowmap ? K.rowmap[G.rowoff + r] : r;
+}
+
+// The cell counter (bucket) of a position: its cell's place in the table, or the scene's
+// out-of-grid bucket.
 __device__ __forceinline__ int cell_of(float x, float y, float z, const StepConsts& K, const SceneGeom& G) {
     int d0 = dim_of(x, G.gmin[0], K.invH, G.dims[0]);
     int d1 = dim_of(y, G.gmin[1], K.invH, G.dims[1]);
     int d2 = dim_of(z, G.gmin[2], K.invH, G.dims[2]);
     bool out = d0 < 0 || d0 >= G.dims[0] || d1 < 0 || d1 >= G.dims[1] || d2 < 0 || d2 >= G.dims[2];
-    return G.coff + (out ? G.ncells : d0 + d1 * G.dims[0] + d2 * G.dims[0] * G.dims[1]);
+    return G.coff + (out ? G.ncells : d0 + row_of(out ? 0 : d1, out ? 0 : d2, K, G) * G.dims[0]);
+}
+
+// computeGid (cuh:156-160): the reference's cell id d0 + dims0 * (d1 + dims1 * d2), -1 outside.
+__device__ __forceinline__ int cell_id_linear(float x, float y, float z, const StepConsts& K, const SceneGeom& G) {
+    int d0 = dim_of(x, G.gmin[0], K.invH, G.dims[0]);
+    int d1 = dim_of(y, G.gmin[1], K.invH, G.dims[1]);
+    int d2 = dim_of(z, G.gmin[2], K.invH, G.dims[2]);
+    bool out = d0 < 0 || d0 >= G.dims[0] || d1 < 0 || d1 >= G.dims[1] || d2 < 0 || d2 >= G.dims[2];
+    return out ? -1 : d0 + (d2 * G.dims[1] + d1) * G.dims[0];
 }
 
 }  // namespace pbf

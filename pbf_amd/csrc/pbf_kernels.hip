@@ -338,7 +338,7 @@ __global__ __launch_bounds__(kBlock) void k_rank_reorder(StepConsts K, const Sce
     vel_out[d] = v;
     xold_out[d] = xo;
     ids_out[d] = id;
-    if (dbg_gid) dbg_gid[id] = inside ? (int32_t)c - G.coff : -1;
+    if (dbg_gid) dbg_gid[id] = cell_id_linear(x.x, x.y, x.z, K, G);  // the reference's id, whatever the row order
 }
 
 // ---------------------------------------------------------------------------------------
@@ -506,7 +506,7 @@ __global__ __launch_bounds__(BS) void k_search(StepConsts K, const SceneGeom* __
             }
             // both indices stay inside the scene's cell table even for an unused run; cell
             // counts stay below 2^28, so their byte offsets fit in 32 bits
-            const int row = use ? G.coff + (cz * G.dims[1] + cy) * G.dims[0] : G.coff;
+            const int row = use ? G.coff + row_of(use ? cy : 0, use ? cz : 0, K, G) * G.dims[0] : G.coff;
             const char* cb = reinterpret_cast<const char*>(cstart);
             const uint32_t lo = *reinterpret_cast<const uint32_t*>(cb + ((uint32_t)(use ? row + rlo : row) << 2));
             const uint32_t hi = *reinterpret_cast<const uint32_t*>(cb + ((uint32_t)(use ? row + rhi + 1 : row) << 2));
