@@ -332,21 +332,32 @@ int enqueue_step(pbf_solver* s, hipEvent_t* ev) {
     return PBF_OK;
 }
 
-int run_step(pbf_solver* s) {
+// Steps per captured multi-step graph: after an even number of steps every double-buffer
+// parity is back where it started, so one such graph serves any number of launches.  Steps
+// inside a graph run back to back; between graph launches the device idles for a few µs.
+constexpr int kGraphSteps = 8;
+
+// k steps (1 or kGraphSteps): eagerly, or from the graph captured for (kind of step, k, parity).
+int run_steps(pbf_solver* s, int k) {
     HIP_TRY(hipSetDevice(s->device));
-    if (!(s->P.flags & PBF_FLAG_GRAPH) || (s->P.flags & PBF_FLAG_DEBUG)) return enqueue_step(s, nullptr);
+    if (!(s->P.flags & PBF_FLAG_GRAPH) || (s->P.flags & PBF_FLAG_DEBUG)) {
+        for (int j = 0; j < k; ++j)
+            if (int rc = enqueue_step(s, nullptr)) return rc;
+        return PBF_OK;
+    }
     // A captured step bakes in which buffer of each pair it reads; vc is unchanged by a step,
-    // so (pc, ic) and the kind of step name the variant.
+    // so (pc, ic), the kind of step and the number of steps name the variant.
     const bool learned = s->learned_passes > 0 && s->mlp;
-    const int key = ((learned ? s->learned_passes : 0) << 2) | (s->pc * 2 + s->ic);
+    const int key = ((learned ? s->learned_passes : 0) << 3) | (k > 1 ? 4 : 0) | (s->pc * 2 + s->ic);
     auto found = s->graphs.find(key);
     if (found == s->graphs.end()) {
         int pc0 = s->pc, vc0 = s->vc, ic0 = s->ic, ll0 = s->lam_last, sd0 = s->steps_done;
         HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_step(s, nullptr);
+        int rc = PBF_OK;
+        for (int j = 0; j < k && rc == PBF_OK; ++j) rc = enqueue_step(s, nullptr);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(s->stream, &g);
-        // capture only recorded the step: restore the parity (also on the error paths below, so
+        // capture only recorded the steps: restore the parity (also on the error paths below, so
         // a failed capture leaves the solver as it was) and launch the graph for real below
         s->pc = pc0;
         s->vc = vc0;
@@ -365,15 +376,29 @@ int run_step(pbf_solver* s) {
         found = s->graphs.emplace(key, ge).first;
     }
     HIP_TRY(hipGraphLaunch(found->second, s->stream));
-    // advance the parity exactly as enqueue_step does
+    // advance the parity exactly as enqueue_step does, once per step
     const int I = learned ? s->learned_passes : s->P.iterations;
-    int a = s->pc ^ 1;
-    for (int it = 0; it < I; ++it) a ^= 1;
-    s->lam_last = I > 0 ? (a ^ 1) : -1;
-    s->pc = a;
-    s->ic ^= 1;
-    s->stats_valid = (s->P.flags & PBF_FLAG_DENSITY_STATS) != 0;
-    s->steps_done++;
+    for (int j = 0; j < k; ++j) {
+        int a = s->pc ^ 1;
+        for (int it = 0; it < I; ++it) a ^= 1;
+        s->lam_last = I > 0 ? (a ^ 1) : -1;
+        s->pc = a;
+        s->ic ^= 1;
+        s->stats_valid = (s->P.flags & PBF_FLAG_DENSITY_STATS) != 0;
+        s->steps_done++;
+    }
+    return PBF_OK;
+}
+
+int run_step(pbf_solver* s) { return run_steps(s, 1); }
+
+// nsteps steps: graphs of kGraphSteps steps while that many remain, then single steps.
+int run_many(pbf_solver* s, int nsteps) {
+    int i = 0;
+    for (; i + kGraphSteps <= nsteps; i += kGraphSteps)
+        if (int rc = run_steps(s, kGraphSteps)) return rc;
+    for (; i < nsteps; ++i)
+        if (int rc = run_steps(s, 1)) return rc;
     return PBF_OK;
 }
 
@@ -674,11 +699,7 @@ int pbf_step(pbf_solver* s, float dt) {
 
 int pbf_advance(pbf_solver* s, int32_t nsteps) {
     if (!s) return set_error(PBF_ERR_ARG, "solver is NULL");
-    for (int i = 0; i < nsteps; ++i) {
-        int rc = run_step(s);
-        if (rc) return rc;
-    }
-    return PBF_OK;
+    return run_many(s, nsteps);
 }
 
 int pbf_synchronize(pbf_solver* s) {
@@ -847,7 +868,7 @@ int pbf_time_steps(pbf_solver* s, int32_t nsteps, float* ms_total) {
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, s->stream));
     int rc = PBF_OK;
-    for (int i = 0; i < nsteps && rc == PBF_OK; ++i) rc = run_step(s);
+    rc = run_many(s, nsteps);
     HIP_TRY(hipEventRecord(e1, s->stream));
     HIP_TRY(hipEventSynchronize(e1));
     float ms = 0.0f;

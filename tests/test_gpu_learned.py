@@ -207,13 +207,14 @@ def test_rejected_learned_step_is_the_plain_step(passes):
     m.set_normalization(np.zeros(16), np.ones(16), 1e3, 0.0)
     lrn.attach_mlp(m)
     lrn.set_learned(passes)
-    plain.advance(6)
-    lrn.advance(6)
-    assert not lrn.last_guess_info()[2]
-    pa, va = plain.get_state()
-    pb, vb = lrn.get_state()
-    assert np.array_equal(pa.view(np.uint32), pb.view(np.uint32))
-    assert np.array_equal(va.view(np.uint32), vb.view(np.uint32))
+    for k in (6, 10):  # single-step graphs, then an 8-step graph and two single steps
+        plain.advance(k)
+        lrn.advance(k)
+        assert not lrn.last_guess_info()[2]
+        pa, va = plain.get_state()
+        pb, vb = lrn.get_state()
+        assert np.array_equal(pa.view(np.uint32), pb.view(np.uint32))
+        assert np.array_equal(va.view(np.uint32), vb.view(np.uint32))
 
 
 def test_trained_network_gain():

@@ -116,6 +116,23 @@ def test_graph_replay_matches_eager():
         assert_bitexact(va, vb, "graph velocities")
 
 
+@pytest.mark.parametrize("iters", [4, 3])
+def test_multi_step_graphs_match_eager(iters):
+    """pbf_advance replays graphs of 8 captured steps while 8 steps remain, then single-step
+    graphs: 8, 19 (the odd remainder moves the next 8-step graph to the other buffer parity) and
+    16 steps give the eager solver's state bit for bit, with an even and an odd iteration count."""
+    sc = scenes.dam_break((24, 24, 24), (0, 0, 0, 1.6, 1.6, 1.6), seed=6)
+    a = Solver.from_scene(sc, flags=0, iterations=iters)
+    b = Solver.from_scene(sc, flags=FLAG_GRAPH, iterations=iters)
+    for k in (8, 19, 16):
+        a.advance(k)
+        b.advance(k)
+        pa, va = a.get_state()
+        pb, vb = b.get_state()
+        assert_bitexact(pa, pb, f"positions after {k}")
+        assert_bitexact(va, vb, f"velocities after {k}")
+
+
 def test_parameter_changes_between_graph_replays():
     """cudaPBFSolve takes the parameters every call: changing the iteration count, the vorticity
     strength and dt between graph-replayed steps rebuilds the captured step, so a graph solver
