@@ -198,10 +198,12 @@ Consts make_consts(const orc_params& P) {
 
 // computeDimID, kernels/cudaPBFSolver.cuh:156-160: int((p - pMin) * invH), truncation toward
 // zero.  Values are clamped to [-3, dim+2] before the conversion (which changes no in-range
-// cell and no 27-cell search result) and NaN maps to a far-away cell.
+// cell and no 27-cell search result).  NaN converts to 0, as the CUDA device's float-to-int
+// conversion (cvt.rzi.s32.f32) returns for NaN, so a NaN coordinate lands in the first cell of
+// its axis and the particle is counted and sorted like any other.
 inline int dim_of(float p, float pmin, float invH, int dim) {
     float f = (p - pmin) * invH;
-    if (!(f == f)) return INT_MIN / 2;
+    if (!(f == f)) return 0;
     if (f < -3.0f) f = -3.0f;
     if (f > (float)(dim + 2)) f = (float)(dim + 2);
     return (int)f;
@@ -468,7 +470,7 @@ void orc_step(orc_state* s, int32_t nsteps) {
                 s->v[i] = {(x.x - s->xo[i].x) / dt, (x.y - s->xo[i].y) / dt, (x.z - s->xo[i].z) / dt};
             }
         }
-        // with no iteration (cudaPBF.cu:329) v keeps its predicted value
+        // with no iteration (the loop of cudaPBF.cu:89-100 does not run) v keeps its predicted value
         // calculateCurl, cu:201-215 (also the density error at the final positions)
 #pragma omp parallel for schedule(static)
         for (int i = 0; i < n; ++i) {
@@ -576,6 +578,24 @@ void orc_get_fields(const orc_state* s, float* lam_first, float* lam_last, float
             curl[3 * i] = s->curl[i].x;
             curl[3 * i + 1] = s->curl[i].y;
             curl[3 * i + 2] = s->curl[i].z;
+        }
+    }
+}
+
+// Vorticity-confinement force eps (N x omega) (cu:237, cuh:146-148) and XSPH term
+// c sum_j -(v_i - v_j) W (cu:229-238) of the last step, before applyVorticityAndViscosity added
+// dt * vort + visc to the velocities (cu:24-29), by particle id.
+void orc_get_vort_visc(const orc_state* s, float* vort, float* visc) {
+    for (int i = 0; i < s->n; ++i) {
+        if (vort) {
+            vort[3 * i] = s->vort[i].x;
+            vort[3 * i + 1] = s->vort[i].y;
+            vort[3 * i + 2] = s->vort[i].z;
+        }
+        if (visc) {
+            visc[3 * i] = s->visc[i].x;
+            visc[3 * i + 1] = s->visc[i].y;
+            visc[3 * i + 2] = s->visc[i].z;
         }
     }
 }

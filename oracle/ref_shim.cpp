@@ -5,7 +5,8 @@
 // PBFSolver::solve (src/solver.cpp:60-157) leaves behind.
 //
 // Used by tests/test_oracle.py to pin the oracle's grid hash, cell table, sorted cell
-// members, neighbour sets, lambda and delta-p against the reference implementation.
+// members, neighbour sets, lambda and delta-p, and (ref_step_probe) its multi-iteration
+// positions, loop velocity, curl and XSPH viscosity against the reference implementation.
 
 // Standard and glm headers first, so their own access specifiers are untouched.
 #include <omp.h>
@@ -185,6 +186,71 @@ int ref_run(const float* params, const float box[6], float* pos, float* vel, int
         vel[3 * i] = particles[i].velocity.x;
         vel[3 * i + 1] = particles[i].velocity.y;
         vel[3 * i + 2] = particles[i].velocity.z;
+    }
+    for (int i = 0; i < n; ++i) delete[] solver->neighbors[i];
+    delete[] solver->neighbors;
+    delete[] solver->grids;
+    delete[] solver->gridParticlesCnt;
+    delete[] solver->particlesGid;
+    delete[] solver->gridParticlesID;
+    delete solver;
+    return 0;
+}
+
+// Runs PBFSolver::solve once on AoS particles (positions and velocities given) and returns what
+// the step leaves behind, for the parts the CPU solver shares with the GPU method:
+//   pos_out    positions after the constraint loop (src/solver.cpp:74-116; the vorticity and
+//              viscosity only change velocities);
+//   vel_loop   the velocity the loop hands to the curl and viscosity sums, (x - x_old) * invDt
+//              (:117-120), recomputed here from the same float operands, so it is bit-identical;
+//   curl       particlesCurl (:122-132, with v_ij = v_i - v_j);
+//   visc       particlesViscosity = c * sum_j -(v_i - v_j) W(r) (:138-150);
+//   vel_out    the final velocities (:152-155);
+//   lambda     the multiplier of the last iteration (:78-95).
+// params: h, restRho, epsilon, k, n, deltaq, radius, c, dt, iterations, vorticity epsilon.
+int ref_step_probe(const float* params, const float box[6], const float* pos, const float* vel, int n, float* pos_out,
+                   float* vel_loop, float* curl, float* visc, float* vel_out, float* lambda) {
+    SolverParameter para;
+    std::memset(&para, 0, sizeof(para));
+    para.h = params[0];
+    para.nParticles = n;
+    para.restRho = params[1];
+    para.epsilon = params[2];
+    para.k = params[3];
+    para.n = params[4];
+    para.deltaq = params[5];
+    para.radius = params[6];
+    para.c = params[7];
+    para.dt = params[8];
+    para.invDt = 1.0f / params[8];
+    para.iterations = (int)params[9];
+    para.vorticityConfinementEpilon = params[10];
+    AABB aabb(glm::vec3(box[0], box[1], box[2]), glm::vec3(box[3], box[4], box[5]));
+    PBFSolver* solver = new PBFSolver(para, aabb);
+    std::memset(solver->gridParticlesCnt, 0, sizeof(int) * solver->gridsCnt);
+    std::vector<Particle> particles(n);
+    for (int i = 0; i < n; ++i) {
+        particles[i].position = glm::vec3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+        particles[i].velocity = glm::vec3(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]);
+    }
+    {
+        QuietStdout q;
+        solver->solve(particles.data());
+    }
+    for (int i = 0; i < n; ++i) {
+        const glm::vec3 x = particles[i].position;
+        const glm::vec3 vl = (x - solver->oldPosition[i]) * para.invDt;
+        const glm::vec3 c = solver->particlesCurl[i];
+        const glm::vec3 vs = solver->particlesViscosity[i];
+        const glm::vec3 v = particles[i].velocity;
+        for (int a = 0; a < 3; ++a) {
+            pos_out[3 * i + a] = x[a];
+            vel_loop[3 * i + a] = vl[a];
+            curl[3 * i + a] = c[a];
+            visc[3 * i + a] = vs[a];
+            vel_out[3 * i + a] = v[a];
+        }
+        lambda[i] = solver->lambda[i];
     }
     for (int i = 0; i < n; ++i) delete[] solver->neighbors[i];
     delete[] solver->neighbors;

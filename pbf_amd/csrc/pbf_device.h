@@ -156,10 +156,13 @@ __device__ __forceinline__ float scorr4(float w, const StepConsts& K) {
 }
 
 // computeDimID, cuh:156-160: truncation toward zero of (p - pMin) * invH, clamped to
-// [-3, dim+2] first (no in-range cell or 27-cell search changes), NaN to a far cell.
+// [-3, dim+2] first (no in-range cell or 27-cell search changes).  NaN gives 0, what the CUDA
+// device's float-to-int conversion returns for NaN: the particle is counted and sorted into the
+// first cell of that axis and, since every distance to it compares false, it neither has nor is
+// a neighbour.
 __device__ __forceinline__ int dim_of(float p, float pmin, float invH, int dim) {
     float f = (p - pmin) * invH;
-    if (!(f == f)) return -0x20000000;
+    if (!(f == f)) return 0;
     f = f < -3.0f ? -3.0f : f;
     f = f > (float)(dim + 2) ? (float)(dim + 2) : f;
     return (int)f;

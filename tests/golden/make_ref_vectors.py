@@ -10,6 +10,12 @@ Outputs (data only):
                     cell start/end, sorted cell members, neighbour lists, lambda and delta-p.
   ref_scan.npz      the reference scan test's serial and Blelloch exclusive scans
                     (test/testPrefixSum.cpp:3-53) of seeded counts in [0, 16).
+  ref_step_probe.npz  one whole PBFSolver::solve (3 iterations, dt = 1/120, XSPH c = 1e-3) on a
+                    seeded 1000-particle block with random velocities, all inside the boundary:
+                    positions after the constraint loop, the loop's velocity (x - x_old) * invDt,
+                    curl, XSPH viscosity and the last iteration's lambda.
+
+    python tests/golden/make_ref_vectors.py [probe] [scan] [step]   (default: all)
 """
 import ctypes as C
 import os
@@ -22,7 +28,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(HERE))
 
-from oracle_lib import fp, ip, ref, up  # noqa: E402
+from oracle_lib import fp, ip, ref, ref_step_probe, up  # noqa: E402
 from pbf_amd import scenes  # noqa: E402
 
 PROBE_PARAMS = dict(h=0.1, restRho=8000.0, epsilon=100.0, k=1e-3, n=4.0, deltaq=0.02, radius=0.03)
@@ -35,7 +41,35 @@ def probe_scene():
     return pos, box
 
 
+STEP_PARAMS = dict(h=0.1, restRho=8000.0, epsilon=100.0, k=1e-3, n=4.0, deltaq=0.02, radius=0.03, c=1e-3,
+                   dt=float(np.float32(1.0 / 120.0)), iterations=3, vorticityConfinementEpilon=0.0)
+
+
+def step_probe_scene():
+    """10^3 block at spacing 0.05 (jitter 0.005) with velocities U(-0.3, 0.3) in a 6.4 m box."""
+    pos = scenes.lattice((10, 10, 10), 0.05, (1.0, 1.0, 1.0), seed=321)
+    vel = (scenes.uniform24(654, pos.size).reshape(pos.shape) * 0.6 - 0.3).astype(np.float32)
+    box = np.array([0, 0, 0, 6.4, 6.4, 6.4], np.float32)
+    return pos, vel, box
+
+
+def write_step_probe():
+    pos, vel, box = step_probe_scene()
+    out = ref_step_probe(STEP_PARAMS, box, pos, vel)
+    prm = np.array([STEP_PARAMS[k] for k in ("h", "restRho", "epsilon", "k", "n", "deltaq", "radius", "c", "dt",
+                                             "iterations", "vorticityConfinementEpilon")], np.float32)
+    np.savez_compressed(os.path.join(HERE, "ref_step_probe.npz"), pos=pos, vel=vel, box=box, params=prm,
+                        pos_out=out["pos"], vel_loop=out["vel_loop"], curl=out["curl"], visc=out["visc"],
+                        lam=out["lambda"])
+
+
 def main():
+    which = set(sys.argv[1:]) or {"probe", "scan", "step"}
+    if "step" in which:
+        write_step_probe()
+        print("wrote ref_step_probe.npz")
+    if not which & {"probe", "scan"}:
+        return
     R = ref()
     pos, box = probe_scene()
     n = pos.shape[0]

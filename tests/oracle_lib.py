@@ -10,7 +10,9 @@ import numpy as np
 from pbf_amd import SolverParameter
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE = os.path.join(ROOT, "oracle", "liboracle.so")
+# PBF_ORACLE_LIB: another build of the oracle sources (the planted-mistake builds of
+# tests/test_oracle_mutants.py)
+ORACLE = os.environ.get("PBF_ORACLE_LIB") or os.path.join(ROOT, "oracle", "liboracle.so")
 REF = os.path.join(ROOT, "oracle", "_ref", "libpbfref.so")
 
 _F = C.POINTER(C.c_float)
@@ -61,6 +63,7 @@ def orc() -> C.CDLL:
             "orc_get_neighbors": (None, [_VP, _I, _I]),
             "orc_get_fields": (None, [_VP, _F, _F, _F, _F]),
             "orc_get_dp": (None, [_VP, _F]),
+            "orc_get_vort_visc": (None, [_VP, _F, _F]),
             "mlp_param_count": (C.c_int, []),
             "mlp_bf16": (C.c_float, [C.c_float]),
             "mlp_forward": (None, [_F, _F, C.c_int, C.c_int, C.POINTER(C.c_double)]),
@@ -91,6 +94,8 @@ def ref() -> C.CDLL:
         L.ref_probe.restype = C.c_int
         L.ref_probe.argtypes = [_F, _F, _F, C.c_int, _I, _I, _I, _I, _I, _F, _F]
         L.ref_grid_cells.restype = C.c_int
+        L.ref_step_probe.restype = C.c_int
+        L.ref_step_probe.argtypes = [_F, _F, _F, _F, C.c_int, _F, _F, _F, _F, _F, _F]
         _ref = L
     return _ref
 
@@ -161,6 +166,13 @@ class Oracle:
         orc().orc_get_dp(self._h, fp(out))
         return out
 
+    def vort_visc(self):
+        """(eps N x omega, c sum_j -(v_i - v_j) W) of the last step, by particle id."""
+        vo = np.empty((self.n, 3), np.float32)
+        vi = np.empty((self.n, 3), np.float32)
+        orc().orc_get_vort_visc(self._h, fp(vo), fp(vi))
+        return vo, vi
+
     def fields(self):
         lf = np.empty(self.n, np.float32)
         ll = np.empty(self.n, np.float32)
@@ -168,6 +180,23 @@ class Oracle:
         cu = np.empty((self.n, 3), np.float32)
         orc().orc_get_fields(self._h, fp(lf), fp(ll), fp(er), fp(cu))
         return lf, ll, er, cu
+
+
+def ref_step_probe(params: dict, box, pos, vel):
+    """One PBFSolver::solve of the reference's CPU solver (src/solver.cpp:60-157) on (pos, vel):
+    dict of positions after the constraint loop, the loop's velocity (x - x_old) * invDt, curl,
+    XSPH viscosity, final velocity and the last iteration's lambda."""
+    prm = np.array([params[k] for k in ("h", "restRho", "epsilon", "k", "n", "deltaq", "radius", "c", "dt",
+                                        "iterations", "vorticityConfinementEpilon")], np.float32)
+    box = np.ascontiguousarray(box, np.float32)
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 3)
+    n = pos.shape[0]
+    out = {k: np.empty((n, 3), np.float32) for k in ("pos", "vel_loop", "curl", "visc", "vel")}
+    out["lambda"] = np.empty(n, np.float32)
+    ref().ref_step_probe(fp(prm), fp(box), fp(pos), fp(vel), n, fp(out["pos"]), fp(out["vel_loop"]), fp(out["curl"]),
+                         fp(out["visc"]), fp(out["vel"]), fp(out["lambda"]))
+    return out
 
 
 def dp_(a):

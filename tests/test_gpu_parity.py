@@ -194,7 +194,7 @@ def test_iteration_counts(iters):
     g, o = pair(scenes.dam_break((16, 16, 16), (0, 0, 0, 1.6, 1.6, 1.6), seed=2), iterations=iters)
     g.step()
     o.step(1)
-    # with no iteration the reference computes no lambda (cudaPBF.cu:329)
+    # with no iteration the reference computes no lambda (the loop of cudaPBF.cu:89-100 does not run)
     compare_step(g, o, check_lambda=iters > 0)
 
 

@@ -1,6 +1,10 @@
-"""Pins the warm-start MLP oracle (oracle/mlp_oracle.cpp): its analytic gradient against
-central finite differences of its own forward pass in FP64, its BF16 rounding against the
-round-to-nearest-even definition, and its Adam step against the closed form of the first step."""
+"""Pins the warm-start MLP oracle (oracle/mlp_oracle.cpp): its FP64 forward pass against the same
+network assembled from torch's float64 linear layers and SiLU and against closed forms (zero
+weights, a single path through the layers, SiLU's identity limit); its analytic gradient against
+torch autograd and central finite differences; its BF16 rounding against the round-to-nearest-even
+definition; and its Adam step against the closed form of the first step."""
+import math
+
 import numpy as np
 import pytest
 
@@ -74,3 +78,101 @@ def test_adam_first_step_closed_form():
     b1, b2 = float(np.float32(0.9)), float(np.float32(0.999))  # the betas arrive as fp32
     np.testing.assert_allclose(m1, (1 - b1) * g.astype(np.float64), rtol=1e-6)
     np.testing.assert_allclose(v1, (1 - b2) * g.astype(np.float64) ** 2, rtol=1e-6)
+
+
+# ---- the forward pass against library routines and closed forms ----------------------------
+def torch_net(p):
+    """The network as the packing defines it (W row-major [out][in], oracle/mlp_oracle.cpp),
+    built from torch's float64 linear layers and SiLU."""
+    import torch
+
+    t = torch.tensor(np.asarray(p, np.float64), requires_grad=True)
+    S = MLP_SLICES
+    W1 = t[S["W1"]].reshape(64, 16)
+    W2 = t[S["W2"]].reshape(64, 64)
+    W3 = t[S["W3"]].reshape(64, 64)
+    W4 = t[S["W4"]].reshape(1, 64)
+
+    def f(x):
+        F = torch.nn.functional
+        h1 = F.silu(F.linear(x, W1, t[S["b1"]]))
+        h2 = F.silu(F.linear(h1, W2, t[S["b2"]]))
+        h3 = F.silu(F.linear(h2, W3, t[S["b3"]]))
+        return F.linear(h3, W4, t[S["b4"]])[:, 0]
+
+    return t, f
+
+
+def test_forward_matches_torch_float64():
+    """Layer order, weight layout and activation: the oracle's FP64 forward equals the same
+    network assembled from torch.nn.functional.linear / silu."""
+    import torch
+
+    rng = np.random.default_rng(11)
+    p = random_params(11)
+    x = rng.standard_normal((40, 16)).astype(np.float32)
+    _, f = torch_net(p)
+    want = f(torch.tensor(x.astype(np.float64))).detach().numpy()
+    got = mlp_forward(p, x, bf16=False)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+
+
+def test_gradient_matches_torch_autograd():
+    import torch
+
+    rng = np.random.default_rng(12)
+    p = random_params(12)
+    x = rng.standard_normal((24, 16)).astype(np.float32)
+    t = rng.standard_normal(24).astype(np.float32)
+    tp, f = torch_net(p)
+    y = f(torch.tensor(x.astype(np.float64)))
+    loss_t = ((y - torch.tensor(t.astype(np.float64))) ** 2).mean()
+    loss_t.backward()
+    loss, g = mlp_loss_grad(p, x, t, bf16=False)
+    assert loss == pytest.approx(loss_t.item(), rel=1e-12)
+    want = tp.grad.numpy()
+    np.testing.assert_allclose(g, want, rtol=1e-10, atol=1e-12 * np.abs(want).max())
+
+
+def test_forward_closed_forms():
+    """All-zero weights give the output bias; with one path through the network (one unit per
+    layer) the output is silu(silu(silu(w x0 + c))) with silu(z) = z / (1 + e^-z), since silu(0) = 0
+    keeps every other unit silent; and silu(z) -> z for large z."""
+    n = orc().mlp_param_count()
+    x = np.zeros((3, 16), np.float32)
+    x[:, 0] = [0.5, -1.25, 2.0]
+    p = np.zeros(n, np.float32)
+    p[MLP_SLICES["b4"]] = 0.375
+    np.testing.assert_array_equal(mlp_forward(p, x, bf16=False), [0.375] * 3)
+
+    def silu(z):
+        return z / (1.0 + math.exp(-z))
+
+    w, c = 1.5, -0.25
+    p = np.zeros(n, np.float32)
+    p[MLP_SLICES["W1"].start + 0 * 16 + 0] = w  # W1[0][0]
+    p[MLP_SLICES["b1"].start] = c
+    p[MLP_SLICES["W2"].start] = 1.0  # W2[0][0]
+    p[MLP_SLICES["W3"].start] = 1.0  # W3[0][0]
+    p[MLP_SLICES["W4"].start] = 1.0  # W4[0]
+    want = [silu(silu(silu(w * float(x0) + c))) for x0 in x[:, 0]]
+    np.testing.assert_allclose(mlp_forward(p, x, bf16=False), want, rtol=1e-14)
+    # a unit in the second slot of layer 1 that W2 reads from its column 1 (W2[0][1])
+    p = np.zeros(n, np.float32)
+    p[MLP_SLICES["W1"].start + 1 * 16 + 3] = w  # W1[1][3]: hidden unit 1 reads input 3
+    p[MLP_SLICES["W2"].start + 0 * 64 + 1] = 1.0  # W2[0][1]
+    p[MLP_SLICES["W3"].start] = 1.0
+    p[MLP_SLICES["W4"].start] = 1.0
+    x2 = np.zeros((2, 16), np.float32)
+    x2[:, 3] = [0.75, -2.0]
+    want = [silu(silu(silu(w * float(v)))) for v in x2[:, 3]]
+    np.testing.assert_allclose(mlp_forward(p, x2, bf16=False), want, rtol=1e-14)
+    # large pre-activations: silu is the identity to double precision
+    p = np.zeros(n, np.float32)
+    p[MLP_SLICES["W1"].start] = 1.0
+    p[MLP_SLICES["W2"].start] = 1.0
+    p[MLP_SLICES["W3"].start] = 1.0
+    p[MLP_SLICES["W4"].start] = 1.0
+    x3 = np.zeros((1, 16), np.float32)
+    x3[0, 0] = 60.0
+    assert mlp_forward(p, x3, bf16=False)[0] == pytest.approx(60.0, rel=1e-20, abs=1e-12)

@@ -184,8 +184,9 @@ def brute_cells(pos, gmin, h, dims):
         d = []
         for a in range(3):
             f = (F32(p[a]) - F32(gmin[a])) * invH
-            d.append(None if np.isnan(f) else int(math.trunc(float(f))) if abs(float(f)) < 1e9 else -(10**9))
-        ok = all(v is not None and 0 <= v < dims[a] for a, v in enumerate(d))
+            # CUDA's float-to-int conversion gives 0 for NaN and saturates huge values
+            d.append(0 if np.isnan(f) else int(math.trunc(float(f))) if abs(float(f)) < 1e9 else -(10**9))
+        ok = all(0 <= v < dims[a] for a, v in enumerate(d))
         out[i] = d[0] + d[1] * dims[0] + d[2] * dims[0] * dims[1] if ok else -1
     return out
 
@@ -211,7 +212,9 @@ def test_cell_ids_brute_force_and_edges():
     gid = np.empty(len(pos), np.int32)
     orc().orc_cell_ids(fp(pos), len(pos), fp(gmin), 0.1, ip(dims), ip(gid))
     assert np.array_equal(gid, brute_cells(pos, gmin, 0.1, dims))
-    assert gid[-7] >= 0 and gid[-6] == -1 and gid[-5] == -1 and gid[-3] == -1 and gid[-2] == -1 and gid[-1] == 0
+    assert gid[-7] >= 0 and gid[-6] == -1 and gid[-5] == -1 and gid[-2] == -1 and gid[-1] == 0
+    # NaN x: cell (0, 4, 3) of the 16 x 8 x 12 grid (the reference GPU's int(NaN) = 0)
+    assert gid[-3] == 0 + 4 * 16 + 3 * 16 * 8
 
 
 def small_scene(seed=9, side=(12, 12, 12), **over):
