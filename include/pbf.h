@@ -103,9 +103,11 @@ int pbf_get_tuning(pbf_solver* s, int32_t out[3]);
 int pbf_set_ordering(pbf_solver* s, int32_t order);
 int pbf_get_ordering(pbf_solver* s, int32_t* out);
 
-/* Autotuner: times every launch shape over `steps` graph-replayed steps, keeps the fastest, then
- * times that shape with the other row order (pbf_set_ordering) and keeps the faster; it restores
- * the state (tuning does not advance the simulation).  best[3] = block, grid, xcd_map; best_ms =
+/* Autotuner: times every launch shape over `steps` steps (rounded up to whole multi-step graphs)
+ * replayed as pbf_advance replays them, keeps the fastest, then times that shape with the other
+ * row order (pbf_set_ordering) and keeps the faster; it restores the state (tuning does not
+ * advance the simulation).  The last step's diagnostics (pbf_get_grid, _neighbors, _fields,
+ * pbf_density_stats) are invalid until the next step.  best[3] = block, grid, xcd_map; best_ms =
  * time per step; the order chosen is pbf_get_ordering's.  Either may be NULL.  No counterpart in
  * the reference. */
 int pbf_autotune(pbf_solver* s, int32_t steps, int32_t* best, float* best_ms);
@@ -128,12 +130,15 @@ int pbf_advance(pbf_solver* s, int32_t nsteps);
 int pbf_get_state(pbf_solver* s, float* pos_xyz, float* vel_xyz);
 
 /* Replace the state (caller's particle order; vel_xyz NULL = zero): the upload of
- * initCUDASolverDataBuffer (kernels/cudaPBF.cu:4-30) into an existing solver. */
+ * initCUDASolverDataBuffer (kernels/cudaPBF.cu:4-30) into an existing solver.  Captured step
+ * graphs stay valid (a caller that uploads every frame keeps replaying them). */
 int pbf_set_state(pbf_solver* s, const float* pos_xyz, const float* vel_xyz);
 
 /* Write positions (and velocities if non-NULL) in the caller's particle order into DEVICE
  * buffers the caller owns, on the solver's stream -- the reference's model where the position
- * buffer belongs to the app (a mapped GL VBO, src/renderapp.cpp:236-249, cudaPBF.cu:7). */
+ * buffer belongs to the app (a mapped GL VBO, src/renderapp.cpp:236-249, cudaPBF.cu:7).  Only
+ * enqueued: pbf_synchronize before another stream or the host reads the buffers (the C++ facade's
+ * exportPositions does). */
 int pbf_export_device(pbf_solver* s, float* dev_pos_xyz, float* dev_vel_xyz);
 
 /* Replaces freeCUDASolverDataBuffer (kernels/cudaPBF.h:11, kernels/cudaPBF.cu:32-48). */
@@ -191,7 +196,8 @@ int pbf_density_stats(pbf_solver* s, float out[3]);
  * clock, src/renderapp.cpp:498-501) ---------------------------------------------------------- */
 
 /* Runs nsteps steps and returns the elapsed time measured with HIP events recorded on the
- * solver's stream around them. */
+ * solver's stream around them.  The step graphs the run replays are captured before the first
+ * event, so the window holds no host capture or instantiation. */
 int pbf_time_steps(pbf_solver* s, int32_t nsteps, float* ms_total);
 
 /* Per-stage times of one step (HIP events between stages, one step, no graph):

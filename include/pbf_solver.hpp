@@ -3,7 +3,8 @@
 // compiles against the MI355X solver by swapping the include.
 //
 //   reference                                      here (namespace pbf_ref)
-//   SolverParameter        include/solverBase.h:16  SolverParameter (= pbf_params, same leading fields)
+//   SolverParameter        include/solverBase.h:16  SolverParameter (pbf_params, same leading fields;
+//                                                   the appended ones default to the reference's constants)
 //   SolverData             include/solverBase.h:45  SolverData (holds the pbf_solver handle)
 //   AABB                   include/AABB.h:6         AABB {vec3 pMin, pMax}
 //   Particle               include/particle.h:9     Particle {vec3 position, velocity}
@@ -41,7 +42,22 @@ struct Particle {
     vec3 velocity;
 };
 
-using SolverParameter = pbf_params;
+// The reference's 13 fields (include/solverBase.h:16-40), zero until the caller sets them, and the
+// constants the reference hard-codes, preset to its values so that a caller that fills only the
+// 13 fields (as src/renderapp.cpp:116-129 does) runs the reference's configuration.
+struct SolverParameter : pbf_params {
+    SolverParameter() : pbf_params{} {
+        gravity[0] = 0.0f;  // kernels/cudaPBFSolver.cu:9
+        gravity[1] = -49.0f;
+        gravity[2] = 0.0f;
+        jitter_coeff = 0.01f;  // RANDOM_OFFSET_COEFF, include/solverBase.h:5
+        grad_rmin = 0.01f;     // kernels/cudaPBFSolver.cuh:107
+        grid_dims[0] = grid_dims[1] = grid_dims[2] = 64;  // UNIFORM_GRID_DIM_SIZE_*, solverBase.h:11-13
+        max_neighbors = 63;    // MAX_NEIGHBOR, solverBase.h:6
+        flags = 0;
+    }
+};
+static_assert(sizeof(SolverParameter) == sizeof(pbf_params), "the facade adds no fields");
 
 struct Error : std::runtime_error {
     int code;
@@ -97,9 +113,12 @@ inline void freeCUDASolverDataBuffer(SolverData& data) {
 }
 
 // Positions (and velocities) in the app's particle order into device buffers it owns -- the
-// reference steps directly in a mapped GL buffer (src/renderapp.cpp:236-249).
+// reference steps directly in a mapped GL buffer (src/renderapp.cpp:236-249).  Returns once the
+// buffers are written, as the reference's synchronous cudaPBFSolve does (cudaPBF.cu:62-113), so
+// the app may draw from them right away.
 inline void exportPositions(SolverData& data, float* dev_pos_xyz, float* dev_vel_xyz = nullptr) {
     check(pbf_export_device(data.solver, dev_pos_xyz, dev_vel_xyz));
+    check(pbf_synchronize(data.solver));
 }
 
 inline void readState(SolverData& data, std::vector<vec3>& position, std::vector<vec3>& velocity) {

@@ -74,6 +74,7 @@ struct pbf_solver {
     float* partial = nullptr;
     int npartial = 0;
     bool stats_valid = false;
+    bool diag_valid = false;     // the step diagnostics describe the current state's last step
     // warm start: features, accumulated multiplier, residual partials, attached network
     float* feat = nullptr;
     float* lam_sum = nullptr;
@@ -328,6 +329,7 @@ int enqueue_step(pbf_solver* s, hipEvent_t* ev) {
     s->vc = v ^ 1;
     s->ic ^= 1;
     s->stats_valid = stats;
+    s->diag_valid = true;
     s->steps_done++;
     return PBF_OK;
 }
@@ -337,33 +339,30 @@ int enqueue_step(pbf_solver* s, hipEvent_t* ev) {
 // inside a graph run back to back; between graph launches the device idles for a few µs.
 constexpr int kGraphSteps = 8;
 
-// k steps (1 or kGraphSteps): eagerly, or from the graph captured for (kind of step, k, parity).
-int run_steps(pbf_solver* s, int k) {
-    HIP_TRY(hipSetDevice(s->device));
-    if (!(s->P.flags & PBF_FLAG_GRAPH) || (s->P.flags & PBF_FLAG_DEBUG)) {
-        for (int j = 0; j < k; ++j)
-            if (int rc = enqueue_step(s, nullptr)) return rc;
-        return PBF_OK;
-    }
-    // A captured step bakes in which buffer of each pair it reads; vc is unchanged by a step,
-    // so (pc, ic), the kind of step and the number of steps name the variant.
+// The executable graph of k steps (1 or kGraphSteps) from the current parity, captured (not
+// launched) on first use.  A captured step bakes in which buffer of each pair it reads; vc is
+// unchanged by a step, so (pc, ic), the kind of step and the number of steps name the variant.
+int graph_for(pbf_solver* s, int k, hipGraphExec_t* out) {
     const bool learned = s->learned_passes > 0 && s->mlp;
     const int key = ((learned ? s->learned_passes : 0) << 3) | (k > 1 ? 4 : 0) | (s->pc * 2 + s->ic);
     auto found = s->graphs.find(key);
     if (found == s->graphs.end()) {
         int pc0 = s->pc, vc0 = s->vc, ic0 = s->ic, ll0 = s->lam_last, sd0 = s->steps_done;
+        const bool st0 = s->stats_valid, dv0 = s->diag_valid;
         HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
         int rc = PBF_OK;
         for (int j = 0; j < k && rc == PBF_OK; ++j) rc = enqueue_step(s, nullptr);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(s->stream, &g);
         // capture only recorded the steps: restore the parity (also on the error paths below, so
-        // a failed capture leaves the solver as it was) and launch the graph for real below
+        // a failed capture leaves the solver as it was)
         s->pc = pc0;
         s->vc = vc0;
         s->ic = ic0;
         s->lam_last = ll0;
         s->steps_done = sd0;
+        s->stats_valid = st0;
+        s->diag_valid = dv0;
         if (rc != PBF_OK) {
             if (g) (void)hipGraphDestroy(g);
             return rc;
@@ -375,8 +374,13 @@ int run_steps(pbf_solver* s, int k) {
         if (e != hipSuccess) return set_error(PBF_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
         found = s->graphs.emplace(key, ge).first;
     }
-    HIP_TRY(hipGraphLaunch(found->second, s->stream));
-    // advance the parity exactly as enqueue_step does, once per step
+    *out = found->second;
+    return PBF_OK;
+}
+
+// The parity and step bookkeeping of k launched steps, exactly as enqueue_step leaves it.
+void advance_parity(pbf_solver* s, int k) {
+    const bool learned = s->learned_passes > 0 && s->mlp;
     const int I = learned ? s->learned_passes : s->P.iterations;
     for (int j = 0; j < k; ++j) {
         int a = s->pc ^ 1;
@@ -385,8 +389,25 @@ int run_steps(pbf_solver* s, int k) {
         s->pc = a;
         s->ic ^= 1;
         s->stats_valid = (s->P.flags & PBF_FLAG_DENSITY_STATS) != 0;
+        s->diag_valid = true;
         s->steps_done++;
     }
+}
+
+bool uses_graphs(const pbf_solver* s) { return (s->P.flags & PBF_FLAG_GRAPH) && !(s->P.flags & PBF_FLAG_DEBUG); }
+
+// k steps (1 or kGraphSteps): eagerly, or from the graph captured for (kind of step, k, parity).
+int run_steps(pbf_solver* s, int k) {
+    HIP_TRY(hipSetDevice(s->device));
+    if (!uses_graphs(s)) {
+        for (int j = 0; j < k; ++j)
+            if (int rc = enqueue_step(s, nullptr)) return rc;
+        return PBF_OK;
+    }
+    hipGraphExec_t ge = nullptr;
+    if (int rc = graph_for(s, k, &ge)) return rc;
+    HIP_TRY(hipGraphLaunch(ge, s->stream));
+    advance_parity(s, k);
     return PBF_OK;
 }
 
@@ -400,6 +421,33 @@ int run_many(pbf_solver* s, int nsteps) {
     for (; i < nsteps; ++i)
         if (int rc = run_steps(s, 1)) return rc;
     return PBF_OK;
+}
+
+// Captures every graph run_many(nsteps) will launch, and launches none, so that a timed
+// run_many does not include host capture and instantiation.
+int prepare_graphs(pbf_solver* s, int nsteps) {
+    if (!uses_graphs(s)) return PBF_OK;
+    HIP_TRY(hipSetDevice(s->device));
+    const int pc0 = s->pc, ic0 = s->ic, ll0 = s->lam_last, sd0 = s->steps_done;
+    const bool st0 = s->stats_valid, dv0 = s->diag_valid;
+    int rc = PBF_OK;
+    hipGraphExec_t ge = nullptr;
+    int i = 0;
+    for (; i + kGraphSteps <= nsteps && rc == PBF_OK; i += kGraphSteps) {
+        rc = graph_for(s, kGraphSteps, &ge);
+        advance_parity(s, kGraphSteps);
+    }
+    for (; i < nsteps && rc == PBF_OK; ++i) {
+        rc = graph_for(s, 1, &ge);
+        advance_parity(s, 1);
+    }
+    s->pc = pc0;
+    s->ic = ic0;
+    s->lam_last = ll0;
+    s->steps_done = sd0;
+    s->stats_valid = st0;
+    s->diag_valid = dv0;
+    return rc;
 }
 
 }  // namespace
@@ -649,9 +697,10 @@ int pbf_set_state(pbf_solver* s, const float* pos_xyz, const float* vel_xyz) {
     launch_import(s->stream, s->n, dp, dv, s->B.pos[0], s->B.vel[0], s->B.ids[0]);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s->stream));
-    drop_graphs(s);
+    // captured steps name buffers and parity only, and the parity is back at (0, 0): they stay
     s->lam_last = -1;
     s->stats_valid = false;
+    s->diag_valid = false;
     return PBF_OK;
 }
 
@@ -745,8 +794,8 @@ int pbf_device_state(pbf_solver* s, pbf_device_view* out) {
 
 int pbf_get_grid(pbf_solver* s, int32_t* gid, int32_t* count, int32_t* cell_start, int32_t* sorted_ids) {
     if (!s) return set_error(PBF_ERR_ARG, "solver is NULL");
-    if (!(s->P.flags & PBF_FLAG_DEBUG) || s->steps_done == 0)
-        return set_error(PBF_ERR_STATE, "needs a step taken with PBF_FLAG_DEBUG");
+    if (!(s->P.flags & PBF_FLAG_DEBUG) || !s->diag_valid)
+        return set_error(PBF_ERR_STATE, "needs a step taken with PBF_FLAG_DEBUG on the current state");
     if (s->nscenes > 1) return set_error(PBF_ERR_STATE, "pbf_get_grid describes single-scene solvers");
     HIP_TRY(hipSetDevice(s->device));
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -786,8 +835,8 @@ int pbf_get_grid(pbf_solver* s, int32_t* gid, int32_t* count, int32_t* cell_star
 
 int pbf_get_neighbors(pbf_solver* s, int32_t* nbr, int32_t* cnt) {
     if (!s) return set_error(PBF_ERR_ARG, "solver is NULL");
-    if (!(s->P.flags & PBF_FLAG_DEBUG) || s->steps_done == 0)
-        return set_error(PBF_ERR_STATE, "needs a step taken with PBF_FLAG_DEBUG");
+    if (!(s->P.flags & PBF_FLAG_DEBUG) || !s->diag_valid)
+        return set_error(PBF_ERR_STATE, "needs a step taken with PBF_FLAG_DEBUG on the current state");
     if (s->n == 0) return PBF_OK;
     HIP_TRY(hipSetDevice(s->device));
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -809,8 +858,8 @@ int pbf_get_neighbors(pbf_solver* s, int32_t* nbr, int32_t* cnt) {
 
 int pbf_get_fields(pbf_solver* s, float* lam_first, float* lam_last, float* err, float* curl_xyz) {
     if (!s) return set_error(PBF_ERR_ARG, "solver is NULL");
-    if (!(s->P.flags & PBF_FLAG_DEBUG) || s->steps_done == 0)
-        return set_error(PBF_ERR_STATE, "needs a step taken with PBF_FLAG_DEBUG");
+    if (!(s->P.flags & PBF_FLAG_DEBUG) || !s->diag_valid)
+        return set_error(PBF_ERR_STATE, "needs a step taken with PBF_FLAG_DEBUG on the current state");
     if (s->n == 0) return PBF_OK;
     HIP_TRY(hipSetDevice(s->device));
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -863,6 +912,9 @@ int pbf_density_stats(pbf_solver* s, float out[3]) {
 int pbf_time_steps(pbf_solver* s, int32_t nsteps, float* ms_total) {
     if (!s || !ms_total) return set_error(PBF_ERR_ARG, "NULL argument");
     HIP_TRY(hipSetDevice(s->device));
+    // every graph the run launches is captured before the first event: the window holds device
+    // work and launches only
+    if (int rc = prepare_graphs(s, nsteps)) return rc;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -972,7 +1024,10 @@ int pbf_time_stage(pbf_solver* s, int32_t stage, int32_t reps, float* ms) {
     (void)hipGraphExecDestroy(ge);
     if (e != hipSuccess) return set_error(PBF_ERR_HIP, "stage timing: %s", hipGetErrorString(e));
     HIP_TRY(hipGetLastError());
+    // the replays rewrote the cell tables, lists and lambda: the step diagnostics are stale
     s->lam_last = -1;
+    s->stats_valid = false;
+    s->diag_valid = false;
     *ms = t / (float)reps;
     return PBF_OK;
 }
@@ -1011,19 +1066,20 @@ int pbf_autotune(pbf_solver* s, int32_t steps, int32_t* best, float* best_ms) {
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    // ms per step of the current shape and order: 2 steps to capture and warm up, then `steps`
+    // ms per step of the current shape and order, stepped as pbf_advance / pbf_time_steps step:
+    // `steps` rounded up to whole multi-step graphs, captured first, after one warm-up graph
+    const int rounds = ((steps + kGraphSteps - 1) / kGraphSteps) * kGraphSteps;
     auto time_config = [&](float& ms) -> int {
         drop_graphs(s);
-        for (int w = 0; w < 2; ++w)
-            if (int r = run_step(s)) return r;
+        if (int r = prepare_graphs(s, rounds)) return r;
+        if (int r = run_many(s, kGraphSteps)) return r;
         if (hipEventRecord(e0, s->stream) != hipSuccess) return set_error(PBF_ERR_HIP, "event");
-        for (int k = 0; k < steps; ++k)
-            if (int r = run_step(s)) return r;
+        if (int r = run_many(s, rounds)) return r;
         if (hipEventRecord(e1, s->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
             return set_error(PBF_ERR_HIP, "event");
         ms = 0.0f;
         (void)hipEventElapsedTime(&ms, e0, e1);
-        ms /= (float)steps;
+        ms /= (float)rounds;
         return PBF_OK;
     };
     for (int bi = 0; bi < 4 && rc == PBF_OK; ++bi)
@@ -1060,6 +1116,11 @@ int pbf_autotune(pbf_solver* s, int32_t steps, int32_t* best, float* best_ms) {
     s->ic = ic0;
     s->steps_done = sd0;
     s->P.flags = flags0;
+    // the tuning steps overwrote the spare buffers, lambda and the cell tables: the last step's
+    // diagnostics (lambda, density statistics, grid, lists) are gone until the next step
+    s->lam_last = -1;
+    s->stats_valid = false;
+    s->diag_valid = false;
     drop_graphs(s);
     hipError_t e = hipMemcpyAsync(s->B.pos[s->pc], sp, sizeof(float4) * np, hipMemcpyDeviceToDevice, s->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(s->B.vel[s->vc], sv, sizeof(float4) * np, hipMemcpyDeviceToDevice, s->stream);
@@ -1483,6 +1544,7 @@ int step_adaptive(pbf_solver* s, int32_t use_mlp, int32_t metric, float tol, int
     s->vc = v ^ 1;
     s->ic ^= 1;
     s->stats_valid = stats;
+    s->diag_valid = true;
     s->steps_done++;
     if (iters_used) *iters_used = it;
     if (resid) *resid = r;
@@ -1582,6 +1644,7 @@ int pbf_snapshot(pbf_solver* s, int32_t save) {
         HIP_TRY(hipMemcpyAsync(id, s->snap_ids, sizeof(uint32_t) * np, hipMemcpyDeviceToDevice, s->stream));
         s->lam_last = -1;
         s->stats_valid = false;
+        s->diag_valid = false;
     }
     HIP_TRY(hipGetLastError());
     return PBF_OK;

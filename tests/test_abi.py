@@ -145,3 +145,30 @@ def test_create_rejects_too_many_particles():
     assert rc == pbf_amd.PBF_ERR_ARG
     assert out.value is None
     assert b"2^28" in lib.pbf_last_error()
+
+
+def test_facade_parameters_default_to_the_reference_constants():
+    """A caller that sets only the reference's 13 SolverParameter fields (src/renderapp.cpp:116-129)
+    gets the constants the reference hard-codes for the appended fields: gravity (0, -49, 0)
+    (kernels/cudaPBFSolver.cu:9), jitter 0.01 (solverBase.h:5), gradient cut-off 0.01
+    (cudaPBFSolver.cuh:107), a 64^3 grid (solverBase.h:11-13) and 63 neighbours (solverBase.h:6)."""
+    src = r"""
+#include <cstdio>
+#include "pbf_solver.hpp"
+using namespace pbf_ref;
+int main() {
+    SolverParameter para;
+    para.h = 1.0; para.nParticles = 8; para.restRho = 6; para.epsilon = 1.0f; para.k = 0.01;
+    para.deltaq = 0.1; para.n = 4; para.c = 0.1f; para.dt = 0.016f; para.invDt = 1.0f / para.dt;
+    para.iterations = 4; para.vorticityConfinementEpilon = 0.1; para.radius = 0.3;
+    std::printf("%g %g %g %g %g %d %d %d %d %u\n", para.gravity[0], para.gravity[1], para.gravity[2],
+                para.jitter_coeff, para.grad_rmin, para.grid_dims[0], para.grid_dims[1], para.grid_dims[2],
+                para.max_neighbors, para.flags);
+    SolverParameter z{};
+    std::printf("%g %d\n", z.h, z.iterations);
+    return 0;
+}
+"""
+    out = compile_and_run(src, "c++").split()
+    assert out[:10] == ["0", "-49", "0", "0.01", "0.01", "64", "64", "64", "63", "0"]
+    assert out[10:] == ["0", "0"]

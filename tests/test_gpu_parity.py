@@ -263,14 +263,19 @@ def test_out_of_grid_and_nan(order):
     pos = sc.pos.copy()
     # grid smaller than the box: particles above y = 0.8 hash outside
     sc.grid_dims = (16, 8, 16)
-    pos[5] = [np.nan, 0.5, 0.5]
+    pos[7] = [0.5, 1.2, 0.5]
+    pos[11] = [0.3, 0.95, 0.7]
+    pos[5] = [np.nan, 0.55, 0.55]
     sc.pos = pos
     g, o = pair(sc)
     g.set_ordering(order)
     g.step()
     o.step(1)
     gg, og = g.grid(), o.grid()
-    assert og[4] < sc.n
+    assert og[4] == sc.n - 2
+    # the NaN coordinate converts to cell 0 of its axis, as CUDA's float-to-int conversion does
+    # (cuh:156-160): cell (0, 5, 5) of the 16 x 8 x 16 grid at its predicted position
+    assert og[0][5] == 0 + 5 * 16 + 5 * 16 * 8
     compare_step(g, o)
 
 
