@@ -235,6 +235,15 @@ int pbf_mlp_forward(pbf_mlp* m, const float* x, int32_t n, float* y);
 /* Mean squared error of y against t over n samples and its gradient (host arrays; grad may be
  * NULL). */
 int pbf_mlp_loss_grad(pbf_mlp* m, const float* x, const float* t, int32_t n, float* grad, float* loss);
+/* pbf_mlp_loss_grad for callers that cannot allocate device memory (serving loops, harnesses
+ * that forbid hipMalloc): no counterpart in the reference.  dev_x [n][16] raw features and
+ * dev_t [n] targets are DEVICE arrays; dev_grad (device) receives the kMlpParams = 9473 gradient
+ * values followed by the sum of squared errors (the loss times n).  Allocates nothing: the
+ * kernels' per-workgroup partial sums go to a fixed library scratch of 256 rows (9.25 MiB per
+ * device, held until the process ends), which bounds a launch to 256 workgroups for any n >= 1.
+ * Results equal pbf_mlp_loss_grad's bit for bit.  Calls that overlap on one device (other
+ * threads, other networks) share that scratch and race: serialise them. */
+int pbf_mlp_loss_grad_noalloc(pbf_mlp* m, const float* dev_x, const float* dev_t, int32_t n, float* dev_grad);
 /* One Adam step with grad (host) or, when NULL, the gradient of the last pbf_mlp_loss_grad. */
 int pbf_mlp_adam_step(pbf_mlp* m, const float* grad, float lr, float beta1, float beta2, float eps);
 /* steps Adam steps on random batches of a device-resident dataset (dev_x [nsamples][16] raw

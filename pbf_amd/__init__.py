@@ -158,6 +158,7 @@ def lib() -> C.CDLL:
         "pbf_mlp_set_normalization": (C.c_int, [_VP, _F, _F, C.c_float, C.c_float]),
         "pbf_mlp_forward": (C.c_int, [_VP, _F, C.c_int32, _F]),
         "pbf_mlp_loss_grad": (C.c_int, [_VP, _F, _F, C.c_int32, _F, _F]),
+        "pbf_mlp_loss_grad_noalloc": (C.c_int, [_VP, _VP, _VP, C.c_int32, _VP]),
         "pbf_mlp_adam_step": (C.c_int, [_VP, _F, C.c_float, C.c_float, C.c_float, C.c_float]),
         "pbf_mlp_train": (C.c_int, [_VP, _VP, _VP, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_uint64, _F]),
         "pbf_attach_mlp": (C.c_int, [_VP, _VP]),
@@ -543,6 +544,11 @@ class Mlp:
         loss = np.zeros(1, np.float32)
         _check(lib().pbf_mlp_loss_grad(self._h, _fp(x), _fp(t), x.shape[0], _fp(g), _fp(loss)))
         return float(loss[0]), g
+
+    def loss_grad_noalloc(self, dev_x: int, dev_t: int, n: int, dev_grad: int):
+        """pbf_mlp_loss_grad_noalloc: device arrays in and out (dev_grad: nparams + 1 floats, the
+        last the sum of squared errors); allocates nothing."""
+        _check(lib().pbf_mlp_loss_grad_noalloc(self._h, dev_x, dev_t, int(n), dev_grad))
 
     def adam_step(self, grad=None, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
         g = None if grad is None else _f32(grad, (self.nparams,))

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1241,7 +1242,7 @@ int pbf_mlp_create(uint64_t seed, pbf_mlp** out) {
     int rc;
     if ((rc = malloc_mlp(m, &m->p, kMlpParams)) || (rc = malloc_mlp(m, &m->m, kMlpParams)) ||
         (rc = malloc_mlp(m, &m->v, kMlpParams)) || (rc = malloc_mlp(m, &m->g, kMlpParams + 1)) ||
-        (rc = malloc_mlp(m, &m->M, 1)) || (rc = malloc_mlp(m, &m->partial, (size_t)256 * (kMlpParams + 1))))
+        (rc = malloc_mlp(m, &m->M, 1)) || (rc = malloc_mlp(m, &m->partial, (size_t)kGradMaxBlocks * (kMlpParams + 1))))
         return fail(rc);
     // He-uniform weights (fan-in), zero biases, small output layer.
     std::vector<float> p(kMlpParams, 0.0f);
@@ -1358,6 +1359,51 @@ int pbf_mlp_loss_grad(pbf_mlp* m, const float* x, const float* t, int32_t n, flo
     if (e != hipSuccess) return set_error(PBF_ERR_HIP, "mlp grad: %s", hipGetErrorString(e));
     if (grad) std::memcpy(grad, g.data(), sizeof(float) * kMlpParams);
     if (loss) *loss = g[kMlpParams] / (float)n;
+    return PBF_OK;
+}
+
+// pbf_mlp_loss_grad_noalloc's scratch: the per-workgroup partial gradient rows of one gradient
+// launch (at most kGradMaxBlocks rows of kMlpParams + 1 floats, 9.25 MiB), one copy per device,
+// held until the process ends.  Host code never uses this name as an address: it uses the
+// per-device pointer hipGetSymbolAddress returns (noalloc_scratch).
+static __device__ __align__(256) float scratch[kGradMaxBlocks * (kMlpParams + 1)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (current), looked up once per device.
+int noalloc_scratch(int device, float** out) {
+    if (device < 0 || device >= kMaxDevices) return set_error(PBF_ERR_ARG, "device index %d out of range", device);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(PBF_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<float*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return PBF_OK;
+}
+}  // namespace
+
+// pbf_mlp_loss_grad on device arrays, with no allocation: the partial rows go to the file-scope
+// scratch (every row the reduction reads is fully written by its workgroup, so nothing needs
+// clearing), samples are taken in order (no index array), and the result stays on the device.
+int pbf_mlp_loss_grad_noalloc(pbf_mlp* m, const float* dev_x, const float* dev_t, int32_t n, float* dev_grad) {
+    if (!m || !dev_x || !dev_t || !dev_grad) return set_error(PBF_ERR_ARG, "NULL argument");
+    if (n < 1) return set_error(PBF_ERR_ARG, "n must be >= 1");
+    const int blocks = mlp_grad_blocks(n);
+    if (blocks > kGradMaxBlocks)
+        return set_error(PBF_ERR_CAPACITY, "%d partial rows exceed the scratch's %d", blocks, kGradMaxBlocks);
+    HIP_TRY(hipSetDevice(m->device));
+    float* part = nullptr;
+    if (int rc = noalloc_scratch(m->device, &part)) return rc;
+    launch_mlp_grad(m->stream, m->M, dev_x, dev_t, nullptr, n, part, dev_grad);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(m->stream));
     return PBF_OK;
 }
 

@@ -32,9 +32,14 @@ struct MlpModel {
     float yscale, yshift;  // multiplier / h^2 = y * yscale + yshift
 };
 
+// Workgroups of a gradient launch: one per 64 samples, at most kGradMaxBlocks (each loops over
+// further chunks); each writes one partial row of kMlpParams + 1 floats.
+constexpr int kGradMaxBlocks = 256;
 int mlp_grad_blocks(int nb);
 void launch_mlp_infer(hipStream_t st, const MlpModel* M, const float* feat, int n, float4* pos, float* y_out,
                       float lam_scale);
+// idx may be NULL (samples 0..nb-1 in order); partial holds mlp_grad_blocks(nb) rows of
+// kMlpParams + 1 floats; grad receives kMlpParams + 1 floats (the last: sum of squared errors).
 void launch_mlp_grad(hipStream_t st, const MlpModel* M, const float* X, const float* T, const uint32_t* idx, int nb,
                      float* partial, float* grad);
 void launch_adam(hipStream_t st, float* p, float* m, float* v, const float* g, float lr, float b1, float b2, float eps,

@@ -140,7 +140,8 @@ __global__ __launch_bounds__(kB) void k_mlp_infer(const MlpModel* __restrict__ M
 
 // ---- training: loss and gradients of a batch -----------------------------------------------
 // partial[blockIdx.x][kMlpParams + 1]: this workgroup's gradient sums (packed as the parameter
-// vector) and its sum of squared errors.  inv_b = 1 / batch size.
+// vector) and its sum of squared errors; every entry is written.  Sample i of the batch is row
+// idx[i] of (X, T), or row i when idx is NULL.  inv_b = 1 / batch size.
 __global__ __launch_bounds__(kB) void k_mlp_grad(const MlpModel* __restrict__ M, const float* __restrict__ X,
                                                  const float* __restrict__ T, const uint32_t* __restrict__ idx, int nb,
                                                  float inv_b, float* __restrict__ partial) {
@@ -161,7 +162,8 @@ __global__ __launch_bounds__(kB) void k_mlp_grad(const MlpModel* __restrict__ M,
 
     for (int base = blockIdx.x * 64; base < nb; base += gridDim.x * 64) {
         load_features(M, [&](int i) -> const float* {
-                          return base + i < nb ? X + (size_t)idx[base + i] * kNumFeatures : nullptr;
+                          return base + i < nb ? X + (size_t)(idx ? idx[base + i] : (uint32_t)(base + i)) * kNumFeatures
+                                               : nullptr;
                       },
                       Xs, XsT);
         __syncthreads();
@@ -186,7 +188,7 @@ __global__ __launch_bounds__(kB) void k_mlp_grad(const MlpModel* __restrict__ M,
             float d = 0.0f;
             if (base + tid < nb) {
                 float y = ypart[0][tid] + ypart[1][tid] + M->b4;
-                float r = y - T[idx[base + tid]];
+                float r = y - T[idx ? idx[base + tid] : (uint32_t)(base + tid)];
                 sse = sse + r * r;
                 d = 2.0f * r * inv_b;
             }
@@ -357,7 +359,7 @@ __global__ __launch_bounds__(kB) void k_batch_indices(uint64_t seed, uint64_t st
 // ---- launch wrappers ----------------------------------------------------------------------
 int mlp_grad_blocks(int nb) {
     int b = (nb + 63) / 64;
-    return b < 1 ? 1 : (b > 256 ? 256 : b);
+    return b < 1 ? 1 : (b > kGradMaxBlocks ? kGradMaxBlocks : b);
 }
 
 void launch_mlp_infer(hipStream_t st, const MlpModel* M, const float* feat, int n, float4* pos, float* y_out,
