@@ -172,12 +172,16 @@ def dam_break(n_side=(32, 32, 32), box=(0, 0, 0, 1.6, 1.6, 1.6), seed=0, spacing
 
 def config_cpu_32k(seed=0) -> Scene:
     """32768 particles, 32^3 lattice in a 1.6 m box (16^3 grid)."""
-    return dam_break((32, 32, 32), (0, 0, 0, 1.6, 1.6, 1.6), seed)
+    sc = dam_break((32, 32, 32), (0, 0, 0, 1.6, 1.6, 1.6), seed)
+    sc.name = "dam break 32768 particles: 32^3 lattice at 0.05 m in a 1.6 m cube, 16^3 cells of h = 0.1 m"
+    return sc
 
 
 def config_262k(seed=1) -> Scene:
     """262144 particles, 64^3 lattice in a 6.4 x 4.0 x 3.2 m box (64 x 40 x 32 grid)."""
-    return dam_break((64, 64, 64), (0, 0, 0, 6.4, 4.0, 3.2), seed)
+    sc = dam_break((64, 64, 64), (0, 0, 0, 6.4, 4.0, 3.2), seed)
+    sc.name = "dam break 262144 particles: 64^3 lattice at 0.05 m in a 6.4 x 4.0 x 3.2 m box, 64 x 40 x 32 cells of h = 0.1 m"
+    return sc
 
 
 def config_1m(seed=2) -> Scene:
@@ -188,7 +192,8 @@ def config_1m(seed=2) -> Scene:
     a = lattice((64, 128, 64), sp, (0.5 * sp, 0.5 * sp, 0.5 * sp), seed)
     x0 = 12.8 - 64 * sp + 0.5 * sp
     b = lattice((64, 128, 64), sp, (x0, 0.5 * sp, 0.5 * sp), seed, seed_offset=1)
-    return _make("double_dam_break_1m", np.concatenate([a, b]), box, p)
+    return _make("double dam break 1048576 particles: two 64 x 128 x 64 blocks at the x-ends of a 12.8 x 6.4 x 6.4 m box, "
+                 "128 x 64 x 64 cells of h = 0.1 m, 8 iterations", np.concatenate([a, b]), box, p)
 
 
 def batch_sizes(count=24, lo=8192, hi=262144, seed=3) -> list:
