@@ -286,9 +286,12 @@ int pbf_set_learned(pbf_solver* s, int32_t passes);
 
 /* Measurement: the average duration of one launch of a step stage (stage 0: neighbour build +
  * iteration-1 lambda, 1: lambda, 2: delta-p + apply, 3: predict + hash, scan, scatter + rank +
- * reorder, 4: an empty kernel of the particle grid's shape, i.e. the cost of one dependent
- * launch, 5: a VALU issue-rate probe of 2048 x 256 threads issuing 65,536 wave64 VALU
- * instructions per wavefront), replayed reps times back to back on the state of the last step,
+ * reorder (the sort stage of a step that hashes for itself), 4: an empty kernel of the particle
+ * grid's shape, i.e. the cost of one dependent launch, 5: a VALU issue-rate probe of 2048 x 256
+ * threads issuing 65,536 wave64 VALU instructions per wavefront, 6: the vorticity pass with the
+ * next step's hash, scan, scatter + rank + reorder (the sort stage of a step whose predecessor
+ * hashed for it, with the vorticity pass), 7: the vorticity pass alone; the vorticity replays
+ * write a spare buffer), replayed reps times back to back on the state of the last step,
  * between HIP events on the solver's stream.  The state is unchanged (the replays write only
  * what the next step rebuilds); the diagnostics of the last step (pbf_get_grid,
  * pbf_get_neighbors, pbf_get_fields) no longer describe it afterwards.  No counterpart in the

@@ -377,15 +377,16 @@ class Solver:
         return float(ms[0])
 
     STAGES = {"neighbors_lambda": 0, "lambda": 1, "delta_apply": 2, "sort_reorder": 3, "launch_floor": 4,
-              "valu_probe": 5}
+              "valu_probe": 5, "vort_hash_sort": 6, "vort_apply": 7}
     VALU_PROBE_INSTR = 2048 * 4 * 16 * 4096  # wave64 VALU instructions of one probe launch (loop body)
 
     def time_stage(self, stage: str, reps: int = 50) -> float:
         """ms per launch of one step stage ("neighbors_lambda", "lambda", "delta_apply",
         "sort_reorder" = predict + hash, scan, scatter + rank + reorder, "launch_floor" = an
-        empty kernel of the particle grid's shape, or "valu_probe" = VALU_PROBE_INSTR multiply
-        and add instructions), replayed back to back on the last step's state (state
-        unchanged)."""
+        empty kernel of the particle grid's shape, "valu_probe" = VALU_PROBE_INSTR multiply and
+        add instructions, "vort_hash_sort" = the vorticity pass with the next step's hash, scan,
+        scatter + rank + reorder, "vort_apply" = the vorticity pass alone), replayed back to back
+        on the last step's state (state unchanged)."""
         ms = np.zeros(1, np.float32)
         _check(lib().pbf_time_stage(self._h, self.STAGES[stage], int(reps), _fp(ms)))
         return float(ms[0])

@@ -76,6 +76,11 @@ struct pbf_solver {
     int npartial = 0;
     bool stats_valid = false;
     bool diag_valid = false;     // the step diagnostics describe the current state's last step
+    // The last step already predicted, hashed and counted the current state for the next step
+    // (cell, local and the cell counters hold it), so the next step starts at the scan.  Cleared,
+    // with the counters zeroed, by anything that changes the state, the step parameters or the
+    // cell order (invalidate_hash).
+    bool hash_ready = false;
     // warm start: features, accumulated multiplier, residual partials, attached network
     float* feat = nullptr;
     float* lam_sum = nullptr;
@@ -197,6 +202,15 @@ void drop_graphs(pbf_solver* s) {
     s->graphs.clear();
 }
 
+// The next step hashes the state itself: the counters the last step filled for it are zeroed
+// (a fill kernel on the solver's stream, ordered before the next step).
+void invalidate_hash(pbf_solver* s) {
+    if (!s->hash_ready) return;
+    (void)hipSetDevice(s->device);
+    launch_fill_u32(s->stream, s->B.count, (size_t)scan_tiles(s->nbuckets) * 1024, 0u);
+    s->hash_ready = false;
+}
+
 // Enqueue one step.  Stage events (7 stage boundaries + start) are recorded when ev != NULL.
 int prof_mark(pbf_solver* s, const char* name) {
     if (!s->prof_events) return PBF_OK;
@@ -272,8 +286,10 @@ int enqueue_step(pbf_solver* s, hipEvent_t* ev) {
     const bool learned = s->learned_passes > 0 && s->mlp;
     if (ev) HIP_TRY(hipEventRecord(ev[0], st));
     PROF("start");
-    launch_predict_hash(st, K, s->geo, B.pos[s->pc], B.vel[s->vc], B);
-    PROF("predict_hash");
+    if (!s->hash_ready) {
+        launch_predict_hash(st, K, s->geo, B.pos[s->pc], B.vel[s->vc], B);
+        PROF("predict_hash");
+    }
     if (ev) HIP_TRY(hipEventRecord(ev[1], st));
     launch_scan(st, K, B);
     PROF("scan");
@@ -321,8 +337,8 @@ int enqueue_step(pbf_solver* s, hipEvent_t* ev) {
     launch_curl(st, K, s->tune, B.pos[a], B.vel[v], B, dbg ? s->dbg_err : nullptr, stats ? s->partial : nullptr);
     PROF("curl");
     if (ev) HIP_TRY(hipEventRecord(ev[6], st));
-    launch_vort_apply(st, K, s->tune, B.pos[a], B.vel[v], B.vel[v ^ 1], B);
-    PROF("vort_apply");
+    launch_vort_apply(st, K, s->geo, s->tune, B.pos[a], B.vel[v], B.vel[v ^ 1], B, true);
+    PROF("vort_apply_hash");
     if (ev) HIP_TRY(hipEventRecord(ev[7], st));
     HIP_TRY(hipGetLastError());
     s->lam_last = I > 0 ? (a ^ 1) : -1;
@@ -331,6 +347,7 @@ int enqueue_step(pbf_solver* s, hipEvent_t* ev) {
     s->ic ^= 1;
     s->stats_valid = stats;
     s->diag_valid = true;
+    s->hash_ready = true;
     s->steps_done++;
     return PBF_OK;
 }
@@ -344,12 +361,15 @@ constexpr int kGraphSteps = 8;
 // launched) on first use.  A captured step bakes in which buffer of each pair it reads; vc is
 // unchanged by a step, so (pc, ic), the kind of step and the number of steps name the variant.
 int graph_for(pbf_solver* s, int k, hipGraphExec_t* out) {
+    // the first step of a graph starts at the hash or, when the last step hashed for it, at the
+    // scan: two variants
     const bool learned = s->learned_passes > 0 && s->mlp;
-    const int key = ((learned ? s->learned_passes : 0) << 3) | (k > 1 ? 4 : 0) | (s->pc * 2 + s->ic);
+    const int key = ((learned ? s->learned_passes : 0) << 4) | (s->hash_ready ? 8 : 0) | (k > 1 ? 4 : 0) |
+                    (s->pc * 2 + s->ic);
     auto found = s->graphs.find(key);
     if (found == s->graphs.end()) {
         int pc0 = s->pc, vc0 = s->vc, ic0 = s->ic, ll0 = s->lam_last, sd0 = s->steps_done;
-        const bool st0 = s->stats_valid, dv0 = s->diag_valid;
+        const bool st0 = s->stats_valid, dv0 = s->diag_valid, hr0 = s->hash_ready;
         HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
         int rc = PBF_OK;
         for (int j = 0; j < k && rc == PBF_OK; ++j) rc = enqueue_step(s, nullptr);
@@ -364,6 +384,7 @@ int graph_for(pbf_solver* s, int k, hipGraphExec_t* out) {
         s->steps_done = sd0;
         s->stats_valid = st0;
         s->diag_valid = dv0;
+        s->hash_ready = hr0;
         if (rc != PBF_OK) {
             if (g) (void)hipGraphDestroy(g);
             return rc;
@@ -391,6 +412,7 @@ void advance_parity(pbf_solver* s, int k) {
         s->ic ^= 1;
         s->stats_valid = (s->P.flags & PBF_FLAG_DENSITY_STATS) != 0;
         s->diag_valid = true;
+        s->hash_ready = true;
         s->steps_done++;
     }
 }
@@ -430,7 +452,7 @@ int prepare_graphs(pbf_solver* s, int nsteps) {
     if (!uses_graphs(s)) return PBF_OK;
     HIP_TRY(hipSetDevice(s->device));
     const int pc0 = s->pc, ic0 = s->ic, ll0 = s->lam_last, sd0 = s->steps_done;
-    const bool st0 = s->stats_valid, dv0 = s->diag_valid;
+    const bool st0 = s->stats_valid, dv0 = s->diag_valid, hr0 = s->hash_ready;
     int rc = PBF_OK;
     hipGraphExec_t ge = nullptr;
     int i = 0;
@@ -448,6 +470,7 @@ int prepare_graphs(pbf_solver* s, int nsteps) {
     s->steps_done = sd0;
     s->stats_valid = st0;
     s->diag_valid = dv0;
+    s->hash_ready = hr0;
     return rc;
 }
 
@@ -659,6 +682,7 @@ int pbf_set_ordering(pbf_solver* s, int32_t order) {
         s->rowmap_host = std::move(map);
     }
     if (order != s->order) {
+        invalidate_hash(s);  // cell ids depend on the row order
         s->order = order;
         drop_graphs(s);
     }
@@ -694,6 +718,7 @@ int pbf_set_state(pbf_solver* s, const float* pos_xyz, const float* vel_xyz) {
     } else {
         launch_fill_u32(s->stream, reinterpret_cast<uint32_t*>(dv), (size_t)s->n * 3, 0u);
     }
+    invalidate_hash(s);
     s->pc = s->vc = s->ic = 0;
     launch_import(s->stream, s->n, dp, dv, s->B.pos[0], s->B.vel[0], s->B.ids[0]);
     HIP_TRY(hipGetLastError());
@@ -730,6 +755,7 @@ int pbf_solve(pbf_solver* s, const pbf_params* para) {
         return set_error(PBF_ERR_CAPACITY, "grid_dims differ from the solver's");
     if (para->max_neighbors != s->cap) return set_error(PBF_ERR_CAPACITY, "max_neighbors differs from the solver's");
     if (std::memcmp(para, &s->P, sizeof(pbf_params)) != 0) {
+        invalidate_hash(s);  // the last step predicted with the old parameters
         s->P = *para;
         drop_graphs(s);
     }
@@ -740,6 +766,7 @@ int pbf_step(pbf_solver* s, float dt) {
     if (!s) return set_error(PBF_ERR_ARG, "solver is NULL");
     if (!(dt > 0.0f)) return set_error(PBF_ERR_ARG, "dt must be > 0");
     if (s->P.dt != dt) {
+        invalidate_hash(s);  // the last step predicted with the old time step
         s->P.dt = dt;
         s->P.invDt = 1.0f / dt;
         drop_graphs(s);
@@ -981,13 +1008,16 @@ int pbf_profile_step(pbf_solver* s, float* ms, const char** names, int32_t max) 
 // pos.w, the free position / velocity / id buffers, x_old; the scan clears the cell counters
 // again), so the state is unchanged.
 int pbf_time_stage(pbf_solver* s, int32_t stage, int32_t reps, float* ms) {
-    if (!s || !ms || reps < 1 || stage < 0 || stage > 5) return set_error(PBF_ERR_ARG, "bad argument");
+    if (!s || !ms || reps < 1 || stage < 0 || stage > 7) return set_error(PBF_ERR_ARG, "bad argument");
     if (s->steps_done < 1 || s->n == 0) return set_error(PBF_ERR_STATE, "run a step first");
     HIP_TRY(hipSetDevice(s->device));
     const StepConsts K = make_consts(s);
     StepBuffers& B = s->B;
     const int a = s->pc;
     const LearnArgs L0{};
+    // the sort-stage replays hash from zeroed counters (and leave them zeroed): the next step
+    // hashes for itself
+    if (stage == 3 || stage == 6) invalidate_hash(s);
     // the reps replays are captured in one graph, as a step runs, so host launch overhead does
     // not enter short stages; one untimed launch of the graph warms up
     HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
@@ -1003,6 +1033,14 @@ int pbf_time_stage(pbf_solver* s, int32_t stage, int32_t reps, float* ms) {
         }
         if (stage == 4) launch_noop(s->stream, s->n);
         if (stage == 5) launch_valu_probe(s->stream, reinterpret_cast<float*>(B.xp));
+        // the vorticity pass writes into the spare staging buffer vp, never the state's velocity
+        if (stage == 6) {
+            launch_vort_apply(s->stream, K, s->geo, s->tune, B.pos[a], B.vel[s->vc ^ 1], B.vp, B, true);
+            launch_scan(s->stream, K, B);
+            launch_sort_reorder(s->stream, K, s->geo, B, B.pos[s->pc], B.vel[s->vc], B.ids[s->ic], B.pos[s->pc ^ 1],
+                                B.vel[s->vc ^ 1], B.ids[s->ic ^ 1], nullptr);
+        }
+        if (stage == 7) launch_vort_apply(s->stream, K, s->geo, s->tune, B.pos[a], B.vel[s->vc ^ 1], B.vp, B, false);
     }
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamEndCapture(s->stream, &g));
@@ -1123,6 +1161,7 @@ int pbf_autotune(pbf_solver* s, int32_t steps, int32_t* best, float* best_ms) {
     s->stats_valid = false;
     s->diag_valid = false;
     drop_graphs(s);
+    invalidate_hash(s);  // the tuning steps hashed their own successors
     hipError_t e = hipMemcpyAsync(s->B.pos[s->pc], sp, sizeof(float4) * np, hipMemcpyDeviceToDevice, s->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(s->B.vel[s->vc], sv, sizeof(float4) * np, hipMemcpyDeviceToDevice, s->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(s->B.ids[s->ic], si, sizeof(uint32_t) * np, hipMemcpyDeviceToDevice, s->stream);
@@ -1530,7 +1569,7 @@ int step_adaptive(pbf_solver* s, int32_t use_mlp, int32_t metric, float tol, int
     const bool dbg = (s->P.flags & PBF_FLAG_DEBUG) != 0;
     const bool stats = dbg || (s->P.flags & PBF_FLAG_DENSITY_STATS);
     const bool features = use_mlp || record;
-    launch_predict_hash(st, K, s->geo, B.pos[s->pc], B.vel[s->vc], B);
+    if (!s->hash_ready) launch_predict_hash(st, K, s->geo, B.pos[s->pc], B.vel[s->vc], B);
     launch_scan(st, K, B);
     launch_sort_reorder(st, K, s->geo, B, B.pos[s->pc], B.vel[s->vc], B.ids[s->ic], B.pos[s->pc ^ 1], B.vel[s->vc ^ 1],
                         B.ids[s->ic ^ 1], dbg ? s->dbg_gid : nullptr);
@@ -1583,7 +1622,7 @@ int step_adaptive(pbf_solver* s, int32_t use_mlp, int32_t metric, float tol, int
     }
     launch_velocity(st, K, B.pos[a], B.vel[v], B);
     launch_curl(st, K, s->tune, B.pos[a], B.vel[v], B, dbg ? s->dbg_err : nullptr, stats ? s->partial : nullptr);
-    launch_vort_apply(st, K, s->tune, B.pos[a], B.vel[v], B.vel[v ^ 1], B);
+    launch_vort_apply(st, K, s->geo, s->tune, B.pos[a], B.vel[v], B.vel[v ^ 1], B, true);
     HIP_TRY(hipGetLastError());
     s->lam_last = a ^ 1;
     s->pc = a;
@@ -1591,6 +1630,7 @@ int step_adaptive(pbf_solver* s, int32_t use_mlp, int32_t metric, float tol, int
     s->ic ^= 1;
     s->stats_valid = stats;
     s->diag_valid = true;
+    s->hash_ready = true;
     s->steps_done++;
     if (iters_used) *iters_used = it;
     if (resid) *resid = r;
@@ -1685,6 +1725,7 @@ int pbf_snapshot(pbf_solver* s, int32_t save) {
         HIP_TRY(hipMemcpyAsync(s->snap_ids, id, sizeof(uint32_t) * np, hipMemcpyDeviceToDevice, s->stream));
         s->snap_valid = true;
     } else {
+        invalidate_hash(s);
         launch_copy_u128(s->stream, s->snap_pos, p, sizeof(float4) * np);
         launch_copy_u128(s->stream, s->snap_vel, v, sizeof(float4) * np);
         HIP_TRY(hipMemcpyAsync(id, s->snap_ids, sizeof(uint32_t) * np, hipMemcpyDeviceToDevice, s->stream));

@@ -111,8 +111,10 @@ void launch_delta_apply(hipStream_t st, const StepConsts& K, const Geometry& GE,
                         float4* pos_next, float4* vel, const StepBuffers& B, int last, int first, const LearnArgs& L);
 void launch_curl(hipStream_t st, const StepConsts& K, const Tuning& T, float4* pos, const float4* vel,
                  const StepBuffers& B, float* err, float* partial);
-void launch_vort_apply(hipStream_t st, const StepConsts& K, const Tuning& T, const float4* pos, const float4* vel,
-                       float4* vel_out, const StepBuffers& B);
+// hash_next: also the next step's predict + hash + count on the final state (the next step then
+// starts at the scan)
+void launch_vort_apply(hipStream_t st, const StepConsts& K, const Geometry& GE, const Tuning& T, const float4* pos,
+                       const float4* vel, float4* vel_out, const StepBuffers& B, bool hash_next);
 // learned pass: accept / reject on the device
 void launch_guess_gate(hipStream_t st, const float* r0, const float* rg, int np, int n, int* gate, float* stats);
 void launch_velocity(hipStream_t st, const StepConsts& K, const float4* pos, float4* vel, const StepBuffers& B);

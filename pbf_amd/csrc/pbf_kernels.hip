@@ -104,20 +104,12 @@ __device__ __forceinline__ void predict(float4& x, float4& v, const StepConsts& 
 // The returned atomic slot gives an (unordered) place inside the cell bucket.
 constexpr int kScanTile = 1024;  // counters per scan tile (256 threads x 4)
 
-__global__ __launch_bounds__(kBlock) void k_predict_hash(StepConsts K, const SceneGeom* __restrict__ geoms,
-                                                         const uint8_t* __restrict__ sid, const float4* __restrict__ pos,
-                                                         const float4* __restrict__ vel, uint32_t* __restrict__ cell,
-                                                         uint32_t* __restrict__ local, uint32_t* __restrict__ count) {
-    int s = xcd_block() * kBlock + threadIdx.x;
-    if (s >= K.n) return;  // whole tail lanes leave; ballots below count the active lanes only
-    const SceneGeom G = geom_of(K, geoms, sid, s);
-    float4 x = pos[s];
-    float4 v = vel[s];
-    predict(x, v, K, G);
-    int c = cell_of(x.x, x.y, x.z, K, G);
-    // Histogram with one atomic per run of equal cells in the wavefront (slots are in the
-    // previous step's cell order, so runs are long): the run head adds the run length and every
-    // lane takes base + its rank in the run.  Any order stays correct; runs only save atomics.
+// Counts this lane's particle into counter c and returns its (unordered) place in the cell's
+// bucket, with one atomic per run of equal cells in the wavefront (slots are in the previous
+// step's cell order, so runs are long): the run head adds the run length and every lane takes
+// base + its rank in the run.  Any order stays correct; runs only save atomics.  The active
+// lanes must be a prefix of the wavefront (slot order, tail lanes inactive).
+__device__ __forceinline__ uint32_t count_in_cell(uint32_t* __restrict__ count, int c) {
     const int lane = threadIdx.x & 63;
     const int prev = __shfl_up(c, 1, 64);
     const bool head = lane == 0 || prev != c;
@@ -131,7 +123,21 @@ __global__ __launch_bounds__(kBlock) void k_predict_hash(StepConsts K, const Sce
     uint32_t base = 0;
     if (head) base = atomicAdd(&count[c], (uint32_t)(run_end - lane));
     base = __shfl(base, hpos, 64);
-    const uint32_t l = base + (uint32_t)(lane - hpos);
+    return base + (uint32_t)(lane - hpos);
+}
+
+__global__ __launch_bounds__(kBlock) void k_predict_hash(StepConsts K, const SceneGeom* __restrict__ geoms,
+                                                         const uint8_t* __restrict__ sid, const float4* __restrict__ pos,
+                                                         const float4* __restrict__ vel, uint32_t* __restrict__ cell,
+                                                         uint32_t* __restrict__ local, uint32_t* __restrict__ count) {
+    int s = xcd_block() * kBlock + threadIdx.x;
+    if (s >= K.n) return;  // whole tail lanes leave; ballots below count the active lanes only
+    const SceneGeom G = geom_of(K, geoms, sid, s);
+    float4 x = pos[s];
+    float4 v = vel[s];
+    predict(x, v, K, G);
+    const int c = cell_of(x.x, x.y, x.z, K, G);
+    const uint32_t l = count_in_cell(count, c);
     cell[s] = (uint32_t)c;
     local[s] = l;
 }
@@ -797,13 +803,18 @@ __global__ __launch_bounds__(BS) void k_curl(StepConsts K, float4* __restrict__ 
 
 // calculateVorticityAndViscosity (cu:217-239) + applyVorticityAndViscosity (cu:24-29): the
 // vorticity gradient from the neighbours' |omega| (pos.w, left by the curl pass), the XSPH sum
-// from the curl pass.
-template <int BS>
-__global__ __launch_bounds__(BS) void k_vort_apply(StepConsts K, const float4* __restrict__ pos,
+// from the curl pass.  HASH: also the next step's updateParticles + calculateParticlesGid
+// (cu:4-13, 31-40) on the final state, which is in registers here: predicted cell and bucket
+// place per slot, counted into the cell counters (the scan of this step has cleared them).  The
+// next step's reorder predicts again from the same values, so both agree bit for bit.
+template <int BS, bool HASH>
+__global__ __launch_bounds__(BS) void k_vort_apply(StepConsts K, const SceneGeom* __restrict__ geoms,
+                                                   const uint8_t* __restrict__ sid, const float4* __restrict__ pos,
                                                    const float4* __restrict__ vel, const float4* __restrict__ omega,
                                                    const float4* __restrict__ xsph, const uint32_t* __restrict__ nbr,
                                                    const uint8_t* __restrict__ ncnt, float4* __restrict__ vel_out,
-                                                   int xcd_map) {
+                                                   uint32_t* __restrict__ cell, uint32_t* __restrict__ local,
+                                                   uint32_t* __restrict__ count, int xcd_map) {
     const BlockRange R((K.n + BS - 1) / BS, xcd_map);
     for (int blk = R.first; blk < R.last; ++blk) {
         const int s = blk * BS + threadIdx.x;
@@ -834,8 +845,17 @@ __global__ __launch_bounds__(BS) void k_vort_apply(StepConsts K, const float4* _
         float cz = ex * wi.y - ey * wi.x;
         float fx = K.vort_eps * cx, fy = K.vort_eps * cy, fz = K.vort_eps * cz;
         float vx = K.visc_c * p.x, vy = K.visc_c * p.y, vz = K.visc_c * p.z;
-        vel_out[s] = make_float4(vi.x + (K.dt * fx + vx), vi.y + (K.dt * fy + vy), vi.z + (K.dt * fz + vz),
-                                 0.0f);
+        float4 vn = make_float4(vi.x + (K.dt * fx + vx), vi.y + (K.dt * fy + vy), vi.z + (K.dt * fz + vz), 0.0f);
+        vel_out[s] = vn;
+        if constexpr (HASH) {
+            const SceneGeom G = geom_of(K, geoms, sid, s);
+            float4 x = xi;
+            predict(x, vn, K, G);
+            const int c = cell_of(x.x, x.y, x.z, K, G);
+            const uint32_t l = count_in_cell(count, c);
+            cell[s] = (uint32_t)c;
+            local[s] = l;
+        }
     }
 }
 
@@ -1075,11 +1095,18 @@ void launch_curl(hipStream_t st, const StepConsts& K, const Tuning& T, float4* p
     }
 }
 
-void launch_vort_apply(hipStream_t st, const StepConsts& K, const Tuning& T, const float4* pos, const float4* vel,
-                       float4* vel_out, const StepBuffers& B) {
+void launch_vort_apply(hipStream_t st, const StepConsts& K, const Geometry& GE, const Tuning& T, const float4* pos,
+                       const float4* vel, float4* vel_out, const StepBuffers& B, bool hash_next) {
     if (K.n <= 0) return;
-    PBF_DISPATCH_BS(T, hipLaunchKernelGGL(k_vort_apply<BS>, dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, vel, B.omega,
-                                          B.xold, B.nbr, B.ncnt, vel_out, T.xcd_map));
+    if (hash_next) {
+        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_vort_apply<BS, true>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
+                                              GE.sid, pos, vel, B.omega, B.xold, B.nbr, B.ncnt, vel_out, B.cell, B.local,
+                                              B.count, T.xcd_map));
+    } else {
+        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_vort_apply<BS, false>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K,
+                                              GE.geoms, GE.sid, pos, vel, B.omega, B.xold, B.nbr, B.ncnt, vel_out, B.cell,
+                                              B.local, B.count, T.xcd_map));
+    }
 }
 
 void launch_velocity(hipStream_t st, const StepConsts& K, const float4* pos, float4* vel, const StepBuffers& B) {

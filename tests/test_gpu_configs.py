@@ -203,3 +203,39 @@ def test_graphs_survive_set_state():
         assert same_bits(gp, ep) and same_bits(gv, ev), f"round {k}"
     g.close()
     e.close()
+
+
+@pytest.mark.parametrize("change", ["set_state", "dt", "ordering", "parameters"])
+def test_changes_between_steps_rehash(change):
+    """A step predicts, hashes and counts the next step's cells with the step's own parameters,
+    state and row order.  After any of those changes, the next step hashes again: the solver
+    equals a fresh one created from the same state with the same settings, bit for bit."""
+    sc = scenes.config_cpu_32k(seed=10)
+    print(f"[case] rehash after {change}", flush=True)
+    a = Solver.from_scene(sc, flags=FLAG_GRAPH)
+    a.advance(5)
+    p, v = a.get_state()
+    para = a.para.copy()
+    if change == "set_state":
+        p = p + np.float32(0.013)
+        a.set_state(p, v)
+    b = Solver(para, sc.box, sc.boundary, p, v)
+    b.set_tuning(*a.get_tuning())
+    if change == "dt":
+        a.step(0.005)
+        b.step(0.005)
+    elif change == "ordering":
+        a.set_ordering(1)
+        b.set_ordering(1)
+    elif change == "parameters":
+        q = para.copy()
+        q.gravity[0] = 2.0
+        a.solve(q)
+        b.solve(q)
+    a.advance(3)
+    b.advance(3)
+    pa, va = a.get_state()
+    pb, vb = b.get_state()
+    a.close()
+    b.close()
+    assert same_bits(pa, pb) and same_bits(va, vb)
