@@ -30,7 +30,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from pbf_amd import FLAG_GRAPH, DeviceBuffer, Solver, learned, lib, scenes  # noqa: E402
+from pbf_amd import FLAG_FAST_MATH, FLAG_GRAPH, DeviceBuffer, Solver, learned, lib, scenes  # noqa: E402
 
 METRIC = "ms/step + particle-steps/s @262k; time to 0.5% density err, learned vs plain"
 HBM_PEAK_GBS = 8000.0
@@ -344,6 +344,22 @@ def main():
     pos, vel = s.get_state()
     finite = bool(np.isfinite(pos).all() and np.isfinite(vel).all())
     s.close()
+
+    # the fast-arithmetic mode (fused multiply-adds, hardware rsqrt in the neighbour terms): same
+    # workload, warm-up and launch shape
+    if batch is not None:
+        f = Solver.batch(batch, flags=flags | FLAG_FAST_MATH)
+    else:
+        f = Solver.from_scene(scene, flags=flags | FLAG_FAST_MATH)
+    f.set_tuning(*tuning)
+    if args.warmup > 0:
+        f.advance(args.warmup)
+    f.synchronize()
+    fast_ms = f.time_steps(args.steps) / max(args.steps, 1)
+    fast = {"ms_per_step": fast_ms, "value": n / (fast_ms * 1e-3), "speedup": ms_step / fast_ms,
+            "delta_apply_launch_ms": f.time_stage("delta_apply", 50), "lambda_launch_ms": f.time_stage("lambda", 50),
+            "parity": "tolerances (one step within 1e-5 of the oracle, 1000-step density statistics within 1%)"}
+    f.close()
     seq = None
     if batch is not None:
         seq_ms = sequential_ms(batch, args.warmup, max(20, args.steps // 5))
@@ -416,6 +432,7 @@ def main():
                          "ms_events_eager": sort_ms_events,
                          "launch_floor_ms": floor_ms,
                          "launch_floor_share": sort_launches * floor_ms / sort_ms if sort_ms > 0 else None},
+        "fast_math": fast,
         "copy_kernel_gbs": copy_gbs,
         "state_finite": finite,
         "cpu_baseline": cpu,

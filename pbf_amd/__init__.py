@@ -38,6 +38,7 @@ PBF_ERR_NOGPU = -5
 FLAG_DEBUG = 1
 FLAG_DENSITY_STATS = 2
 FLAG_GRAPH = 4
+FLAG_FAST_MATH = 8
 
 
 class PBFError(RuntimeError):

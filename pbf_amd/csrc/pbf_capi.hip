@@ -140,6 +140,8 @@ StepConsts make_consts(const pbf_solver* s) {
     K.vort_eps = P.vorticityConfinementEpilon;
     K.jitter = P.jitter_coeff;
     K.rmin = P.grad_rmin;
+    K.rmin2 = P.grad_rmin * P.grad_rmin;
+    K.fast = (P.flags & PBF_FLAG_FAST_MATH) ? 1 : 0;
     for (int a = 0; a < 3; ++a) {
         K.g[a] = P.gravity[a];
         K.gmin[a] = s->gs[a];

@@ -7,6 +7,9 @@
 //     operations), not from sqrt and a division;
 //   * integer powers by repeated multiplication;
 //   * the jitter's sin is pbf_sin below, not the device libm's.
+// The fast mode (StepConsts::fast, PBF_FLAG_FAST_MATH) leaves that contract in the neighbour
+// terms of the constraint passes: fused multiply-adds and the hardware reciprocal square root
+// (the *_t<FAST> helpers below); its results are held to tolerances instead of bits.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -28,6 +31,8 @@ struct SceneGeom {
 // Everything a step kernel needs, passed by value as one kernel argument.
 struct StepConsts {
     float h, h2, c6, cs, wdq, inv_wdq, k, scorr_n, restRho, eps, dt, visc_c, vort_eps, jitter, rmin;
+    float rmin2;  // rmin * rmin (the fast mode's cut-off test)
+    int fast;     // PBF_FLAG_FAST_MATH: the constraint kernels' FAST instances
     int n_int;  // exponent of s_corr when it is a small non-negative integer, else -1
     float g[3];
     float gmin[3];
@@ -126,6 +131,35 @@ __device__ __forceinline__ float spiky_scale(float r2, const StepConsts& K) {
     const float hl = fmaxf(K.h - len, 0.0f);
     const float t = (K.cs * (hl * hl)) * inv;
     return len < K.rmin ? 0.0f : t;
+}
+
+// a * b + c: two IEEE operations (the bit-exact contract), or one fused multiply-add (FAST).
+template <bool FAST>
+__device__ __forceinline__ float madd(float a, float b, float c) {
+    if constexpr (FAST) return __builtin_fmaf(a, b, c);
+    return a * b + c;
+}
+
+// r.r: left to right (bit-exact contract) or with fused multiply-adds (FAST).
+template <bool FAST>
+__device__ __forceinline__ float dot3_t(float x, float y, float z) {
+    if constexpr (FAST) return __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x));
+    return x * x + y * y + z * z;
+}
+
+// spiky_scale with the hardware reciprocal square root (v_rsq_f32, about 1 ulp) in FAST mode.
+// The cut-off |r| < rmin is tested as r.r < rmin^2, which also keeps r.r = 0 (1/0 = inf) out of
+// the result.
+template <bool FAST>
+__device__ __forceinline__ float spiky_scale_t(float r2, const StepConsts& K) {
+    if constexpr (FAST) {
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        const float len = r2 * inv;
+        const float hl = fmaxf(K.h - len, 0.0f);
+        const float t = (K.cs * (hl * hl)) * inv;
+        return r2 < K.rmin2 ? 0.0f : t;
+    }
+    return spiky_scale(r2, K);
 }
 
 // b^e for 0 <= e < 128: the multiplications of square-and-multiply from the low bit up (the

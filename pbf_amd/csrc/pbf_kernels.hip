@@ -352,14 +352,19 @@ __global__ __launch_bounds__(kBlock) void k_rank_reorder(StepConsts K, const Sce
 struct LambdaAcc {
     float rho, gms, gx, gy, gz;
     __device__ __forceinline__ void init() { rho = gms = gx = gy = gz = 0.0f; }
+    template <bool FAST>
     __device__ __forceinline__ void add(float rx, float ry, float rz, float r2, const StepConsts& K) {
         rho = rho + poly6(r2, K);
-        float t = spiky_scale(r2, K);
+        float t = spiky_scale_t<FAST>(r2, K);
         float ax = rx * t, ay = ry * t, az = rz * t;
         gx = gx + ax;
         gy = gy + ay;
         gz = gz + az;
-        gms = gms + (ax * ax + ay * ay + az * az);
+        if constexpr (FAST) {
+            gms = __builtin_fmaf(az, az, __builtin_fmaf(ay, ay, __builtin_fmaf(ax, ax, gms)));
+        } else {
+            gms = gms + (ax * ax + ay * ay + az * az);
+        }
     }
     __device__ __forceinline__ float lambda(const StepConsts& K) const {
         float Ci = rho / K.restRho - 1.0f;
@@ -575,7 +580,7 @@ __global__ __launch_bounds__(BS) void k_search(StepConsts K, const SceneGeom* __
 // to pos.w and to lam1 (a warm-start feature of pass 2).
 // With L.feat, C at the predicted positions goes to feature slot 10; with L.resid the workgroup
 // sums of max(C, 0) and |C| at the predicted positions.
-template <int BS>
+template <int BS, bool FAST>
 __global__ __launch_bounds__(BS) void k_lambda_first(StepConsts K, float4* __restrict__ pos,
                                                      const uint32_t* __restrict__ nbr, const uint8_t* __restrict__ ncnt,
                                                      float* __restrict__ lam1, float* __restrict__ dbg_lam,
@@ -591,8 +596,7 @@ __global__ __launch_bounds__(BS) void k_lambda_first(StepConsts K, float4* __res
         A.init();
         for_neighbors(nbr, s, K.nstride, cnt, pos, [&](uint32_t, const float4& xj) {
             float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-            float r2 = rx * rx + ry * ry + rz * rz;
-            A.add(rx, ry, rz, r2, K);
+            A.template add<FAST>(rx, ry, rz, dot3_t<FAST>(rx, ry, rz), K);
         });
         const float lam = A.lambda(K);
         reinterpret_cast<float*>(pos + s)[3] = lam;
@@ -608,7 +612,7 @@ __global__ __launch_bounds__(BS) void k_lambda_first(StepConsts K, float4* __res
 
 // calculateLambda (cu:156-180) over the frozen list, iterations 2..I.  With L.resid, the
 // workgroup sums of max(C_i, 0) and |C_i| at the current positions are written as well.
-template <int BS, bool FEAT>
+template <int BS, bool FEAT, bool FAST>
 __global__ __launch_bounds__(BS) void k_lambda(StepConsts K, const SceneGeom* __restrict__ geoms,
                                                const uint8_t* __restrict__ sid, float4* __restrict__ pos,
                                                const uint32_t* __restrict__ nbr, const uint8_t* __restrict__ ncnt,
@@ -625,8 +629,7 @@ __global__ __launch_bounds__(BS) void k_lambda(StepConsts K, const SceneGeom* __
         A.init();
         for_neighbors(nbr, s, K.nstride, cnt, pos, [&](uint32_t, const float4& xj) {
             float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-            float r2 = rx * rx + ry * ry + rz * rz;
-            A.add(rx, ry, rz, r2, K);
+            A.template add<FAST>(rx, ry, rz, dot3_t<FAST>(rx, ry, rz), K);
         });
         const float lam = A.lambda(K);
         reinterpret_cast<float*>(pos + s)[3] = lam;
@@ -675,7 +678,7 @@ __global__ __launch_bounds__(BS) void k_lambda(StepConsts K, const SceneGeom* __
 // LS: the multipliers come from LA.lam_src (n + 1 entries, the padding slot's 0 included)
 // instead of pos.w: the fallback pass after a rejected learned guess reads the kept lambda 2
 // there, so pos.w (the guess) needs no restore.
-template <int BS, int SN, bool LS>
+template <int BS, int SN, bool LS, bool FAST>
 __global__ __launch_bounds__(BS) void k_delta_apply(StepConsts K, const SceneGeom* __restrict__ geoms,
                                                     const uint8_t* __restrict__ sid, const float4* __restrict__ pos,
                                                     float4* __restrict__ pos_next, const uint32_t* __restrict__ nbr,
@@ -694,14 +697,14 @@ __global__ __launch_bounds__(BS) void k_delta_apply(StepConsts K, const SceneGeo
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
         for_neighbors(nbr, s, K.nstride, cnt, pos, [&](uint32_t q, const float4& xj) {
             float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-            float r2 = rx * rx + ry * ry + rz * rz;
+            float r2 = dot3_t<FAST>(rx, ry, rz);
             float w = poly6(r2, K);
-            float t = spiky_scale(r2, K);
+            float t = spiky_scale_t<FAST>(r2, K);
             const float lam_j = LS ? LA.lam_src[q] : xj.w;
             float a = (lam_i + lam_j) + (SN == 4 ? scorr4(w, K) : scorr(w, K));
-            ax = ax + a * (rx * t);
-            ay = ay + a * (ry * t);
-            az = az + a * (rz * t);
+            ax = madd<FAST>(a, rx * t, ax);
+            ay = madd<FAST>(a, ry * t, ay);
+            az = madd<FAST>(a, rz * t, az);
         });
         float x = xi.x + ax / K.restRho;
         float y = xi.y + ay / K.restRho;
@@ -725,7 +728,7 @@ __global__ __launch_bounds__(BS) void k_delta_apply(StepConsts K, const SceneGeo
 // velocity is updated).  The XSPH term (v_j - v_i) W is the negated (v_i - v_j) W of the
 // reference, and x - (-a) = x + a exactly, so the sum is the reference's bit for bit.  This pass
 // reads only .xyz of its neighbours' positions.
-template <int BS, bool STATS>
+template <int BS, bool STATS, bool FAST>
 __global__ __launch_bounds__(BS) void k_curl(StepConsts K, float4* __restrict__ pos, const float4* __restrict__ vel,
                                              const uint32_t* __restrict__ nbr, const uint8_t* __restrict__ ncnt,
                                              float4* __restrict__ omega, float4* __restrict__ xsph,
@@ -753,16 +756,22 @@ __global__ __launch_bounds__(BS) void k_curl(StepConsts K, float4* __restrict__ 
                 const float4 xj = xs[u], vj = vs[u];
                 float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
                 float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-                float r2 = rx * rx + ry * ry + rz * rz;
-                float t = spiky_scale(r2, K);
+                float r2 = dot3_t<FAST>(rx, ry, rz);
+                float t = spiky_scale_t<FAST>(r2, K);
                 float gx = rx * t, gy = ry * t, gz = rz * t;
-                wx = wx + (uy * gz - uz * gy);
-                wy = wy + (uz * gx - ux * gz);
-                wz = wz + (ux * gy - uy * gx);
+                if constexpr (FAST) {
+                    wx = __builtin_fmaf(uy, gz, __builtin_fmaf(-uz, gy, wx));
+                    wy = __builtin_fmaf(uz, gx, __builtin_fmaf(-ux, gz, wy));
+                    wz = __builtin_fmaf(ux, gy, __builtin_fmaf(-uy, gx, wz));
+                } else {
+                    wx = wx + (uy * gz - uz * gy);
+                    wy = wy + (uz * gx - ux * gz);
+                    wz = wz + (ux * gy - uy * gx);
+                }
                 const float w = poly6(r2, K);
-                px = px + ux * w;
-                py = py + uy * w;
-                pz = pz + uz * w;
+                px = madd<FAST>(ux, w, px);
+                py = madd<FAST>(uy, w, py);
+                pz = madd<FAST>(uz, w, pz);
                 if constexpr (STATS) rho = rho + w;
             }
         }
@@ -807,7 +816,7 @@ __global__ __launch_bounds__(BS) void k_curl(StepConsts K, float4* __restrict__ 
 // (cu:4-13, 31-40) on the final state, which is in registers here: predicted cell and bucket
 // place per slot, counted into the cell counters (the scan of this step has cleared them).  The
 // next step's reorder predicts again from the same values, so both agree bit for bit.
-template <int BS, bool HASH>
+template <int BS, bool HASH, bool FAST>
 __global__ __launch_bounds__(BS) void k_vort_apply(StepConsts K, const SceneGeom* __restrict__ geoms,
                                                    const uint8_t* __restrict__ sid, const float4* __restrict__ pos,
                                                    const float4* __restrict__ vel, const float4* __restrict__ omega,
@@ -828,11 +837,11 @@ __global__ __launch_bounds__(BS) void k_vort_apply(StepConsts K, const SceneGeom
         for_neighbors(nbr, s, K.nstride, cnt, pos, [&](uint32_t, const float4& xj) {
             const float wl = xj.w;  // |omega_j|
             float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-            float r2 = rx * rx + ry * ry + rz * rz;
-            float t = spiky_scale(r2, K);
-            ex = ex + wl * (rx * t);
-            ey = ey + wl * (ry * t);
-            ez = ez + wl * (rz * t);
+            float r2 = dot3_t<FAST>(rx, ry, rz);
+            float t = spiky_scale_t<FAST>(r2, K);
+            ex = madd<FAST>(wl, rx * t, ex);
+            ey = madd<FAST>(wl, ry * t, ey);
+            ez = madd<FAST>(wl, rz * t, ez);
         });
         float el = sqrtf(ex * ex + ey * ey + ez * ez);
         if ((double)fabsf(el) > 1e-3) {
@@ -1031,6 +1040,16 @@ void launch_sort_reorder(hipStream_t st, const StepConsts& K, const Geometry& GE
                        B.cell, B.cstart, vel_in, pos_in, pos_out, vel_out, B.xold, ids_out, dbg_gid);
 }
 
+// The FAST instances (fused multiply-adds, hardware rsqrt) when the step constants ask for them.
+#define PBF_DISPATCH_FAST(K, CALL)                 \
+    if ((K).fast) {                                \
+        constexpr bool FAST = true;                \
+        CALL;                                      \
+    } else {                                       \
+        constexpr bool FAST = false;               \
+        CALL;                                      \
+    }
+
 #define PBF_DISPATCH_BS(T, CALL)            \
     switch ((T).block) {                    \
         case 64: { constexpr int BS = 64; CALL; } break;   \
@@ -1049,19 +1068,19 @@ void launch_neighbors_lambda(hipStream_t st, const StepConsts& K, const Geometry
         PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_search<BS, false>), dim3(ngrid_search(T, K.n)), dim3(BS), 0, st, K,
                                               GE.geoms, GE.sid, pos, B.cstart, B.nbr, B.ncnt, T.xcd_map));
     }
-    PBF_DISPATCH_BS(T, hipLaunchKernelGGL(k_lambda_first<BS>, dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, B.nbr,
-                                          B.ncnt, B.lam1, dbg_lam, L, T.xcd_map));
+    PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_lambda_first<BS, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, B.nbr,
+                                          B.ncnt, B.lam1, dbg_lam, L, T.xcd_map)));
 }
 
 void launch_lambda(hipStream_t st, const StepConsts& K, const Geometry& GE, const Tuning& T, float4* pos,
                    const StepBuffers& B, const LearnArgs& L) {
     if (K.n <= 0) return;
     if (L.feat) {
-        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_lambda<BS, true>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
-                                              GE.sid, pos, B.nbr, B.ncnt, L, T.xcd_map));
+        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_lambda<BS, true, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
+                                              GE.sid, pos, B.nbr, B.ncnt, L, T.xcd_map)));
     } else {
-        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_lambda<BS, false>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
-                                              GE.sid, pos, B.nbr, B.ncnt, L, T.xcd_map));
+        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_lambda<BS, false, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
+                                              GE.sid, pos, B.nbr, B.ncnt, L, T.xcd_map)));
     }
 }
 
@@ -1069,17 +1088,17 @@ void launch_delta_apply(hipStream_t st, const StepConsts& K, const Geometry& GE,
                         float4* pos_next, float4* vel, const StepBuffers& B, int last, int first, const LearnArgs& L) {
     if (K.n <= 0) return;
     if (L.lam_src) {  // the fallback after a rejected learned guess (rare): the generic s_corr kernel
-        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_delta_apply<BS, 0, true>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K,
+        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_delta_apply<BS, 0, true, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K,
                                               GE.geoms, GE.sid, pos, pos_next, B.nbr, B.ncnt, B.xold, vel, last, first,
-                                              L, T.xcd_map));
+                                              L, T.xcd_map)));
     } else if (K.n_int == 4) {
-        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_delta_apply<BS, 4, false>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K,
+        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_delta_apply<BS, 4, false, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K,
                                               GE.geoms, GE.sid, pos, pos_next, B.nbr, B.ncnt, B.xold, vel, last, first,
-                                              L, T.xcd_map));
+                                              L, T.xcd_map)));
     } else {
-        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_delta_apply<BS, 0, false>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K,
+        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_delta_apply<BS, 0, false, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K,
                                               GE.geoms, GE.sid, pos, pos_next, B.nbr, B.ncnt, B.xold, vel, last, first,
-                                              L, T.xcd_map));
+                                              L, T.xcd_map)));
     }
 }
 
@@ -1087,11 +1106,11 @@ void launch_curl(hipStream_t st, const StepConsts& K, const Tuning& T, float4* p
                  const StepBuffers& B, float* err, float* partial) {
     if (K.n <= 0) return;
     if (err || partial) {
-        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_curl<BS, true>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, vel,
-                                              B.nbr, B.ncnt, B.omega, B.xold, err, partial, T.xcd_map));
+        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_curl<BS, true, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, vel,
+                                              B.nbr, B.ncnt, B.omega, B.xold, err, partial, T.xcd_map)));
     } else {
-        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_curl<BS, false>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, vel,
-                                              B.nbr, B.ncnt, B.omega, B.xold, err, partial, T.xcd_map));
+        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_curl<BS, false, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, vel,
+                                              B.nbr, B.ncnt, B.omega, B.xold, err, partial, T.xcd_map)));
     }
 }
 
@@ -1099,13 +1118,13 @@ void launch_vort_apply(hipStream_t st, const StepConsts& K, const Geometry& GE, 
                        const float4* vel, float4* vel_out, const StepBuffers& B, bool hash_next) {
     if (K.n <= 0) return;
     if (hash_next) {
-        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_vort_apply<BS, true>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
+        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_vort_apply<BS, true, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
                                               GE.sid, pos, vel, B.omega, B.xold, B.nbr, B.ncnt, vel_out, B.cell, B.local,
-                                              B.count, T.xcd_map));
+                                              B.count, T.xcd_map)));
     } else {
-        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_vort_apply<BS, false>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K,
+        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_vort_apply<BS, false, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K,
                                               GE.geoms, GE.sid, pos, vel, B.omega, B.xold, B.nbr, B.ncnt, vel_out, B.cell,
-                                              B.local, B.count, T.xcd_map));
+                                              B.local, B.count, T.xcd_map)));
     }
 }
 

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from oracle_lib import Oracle
-from pbf_amd import FLAG_DEBUG, FLAG_DENSITY_STATS, FLAG_GRAPH, Solver, scenes
+from pbf_amd import FLAG_DEBUG, FLAG_DENSITY_STATS, FLAG_FAST_MATH, FLAG_GRAPH, Solver, scenes
 
 pytestmark = pytest.mark.gpu
 
@@ -421,73 +421,123 @@ def test_20_steps_262k_bitexact():
     assert_bitexact(gv, ov, "velocities")
 
 
-@pytest.mark.timeout(900)
-def test_1000_steps_density_statistics_262k():
-    """The 262k dam break over 1000 steps (the oracle takes about a minute on 16 host threads):
-    density-error statistics within 1% every 250 steps, and the final state bit for bit."""
-    print("[case] 262k x 1000 steps", flush=True)
-    g, o = pair(scenes.config_262k(), flags=FLAG_DENSITY_STATS)
-    for k in range(4):
-        g.advance(250)
-        o.step(250)
-        st = g.density_stats()
+def oracle_stats(sc, steps, every):
+    """The oracle's density-error statistics (mean |err|, mean max(err, 0), max err) every `every`
+    steps over `steps` steps, and its final state."""
+    o = Oracle.from_scene(sc)
+    stats = []
+    for _ in range(steps // every):
+        o.step(every)
         err = o.fields()[2].astype(np.float64)
-        ref = (np.abs(err).mean(), np.maximum(err, 0).mean(), err.max())
-        np.testing.assert_allclose(st, ref, rtol=1e-2, err_msg=f"step {250 * (k + 1)}")
-        print(f"  step {250 * (k + 1)}: mean |err| {st[0]:.5f} compression {st[1]:.5f} max {st[2]:.4f}", flush=True)
+        stats.append((np.abs(err).mean(), np.maximum(err, 0).mean(), err.max()))
+    return np.array(stats), o.get_state()
+
+
+@pytest.fixture(scope="module")
+def oracle_262k_1000():
+    """The oracle's 262k dam break over 1000 steps (about a minute on 16 host threads), sampled
+    every 50 steps."""
+    return oracle_stats(scenes.config_262k(), 1000, 50)
+
+
+@pytest.mark.timeout(900)
+def test_1000_steps_density_statistics_262k(oracle_262k_1000):
+    """The 262k dam break over 1000 steps: density-error statistics within 1% every 50 steps, and
+    the final state bit for bit."""
+    print("[case] 262k x 1000 steps", flush=True)
+    ref_stats, (op, ov) = oracle_262k_1000
+    g = Solver.from_scene(scenes.config_262k(), flags=FLAG_DENSITY_STATS)
+    for k in range(len(ref_stats)):
+        g.advance(50)
+        st = g.density_stats()
+        np.testing.assert_allclose(st, ref_stats[k], rtol=1e-2, err_msg=f"step {50 * (k + 1)}")
+        if (k + 1) % 5 == 0:
+            print(f"  step {50 * (k + 1)}: mean |err| {st[0]:.5f} compression {st[1]:.5f} max {st[2]:.4f}", flush=True)
     gp, gv = g.get_state()
-    op, ov = o.get_state()
     assert_bitexact(gp, op, "positions after 1000 steps")
     assert_bitexact(gv, ov, "velocities after 1000 steps")
 
 
-def test_gpu_against_reference_golden_vectors():
-    """The GPU step against vectors recorded from the reference's own CPU solver on the same
-    input (tests/golden/ref_probe_3k.npz: 3375 particles, zero velocity and gravity, one
-    iteration): cell ids, cell starts / ends, per-cell member sets, neighbour sets exactly; lambda
-    of the iteration within 2e-5 and delta-p within 5e-5 of their largest magnitude.  The
-    reference's CPU scatter orders a cell's members by descending id, this solver by ascending
-    id, so members and neighbours are compared as sets."""
-    import os
+# ---- fast arithmetic (PBF_FLAG_FAST_MATH): tolerances instead of bits ---------------------------
+def rel_diff(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / np.abs(b).max())
 
-    from pbf_amd import SolverParameter
 
-    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_probe_3k.npz"))
-    pos, box, prm = g["pos"], g["box"], g["params"]
-    p = scenes.default_params(
-        h=float(prm[0]), restRho=float(prm[1]), epsilon=float(prm[2]), k=float(prm[3]), n=float(prm[4]),
-        deltaq=float(prm[5]), radius=float(prm[6]), gravity=(0.0, 0.0, 0.0), iterations=1, grad_rmin=0.01,
-        c=0.0, vorticityConfinementEpilon=0.0,
-    )
-    para = SolverParameter.from_dict(p, len(pos), (64, 64, 64), FLAG_DEBUG)
-    bd = np.concatenate([box[:3] + np.float32(prm[6]), box[3:] - np.float32(prm[6])]).astype(np.float32)
-    print(f"[case] reference probe, {len(pos)} particles", flush=True)
-    s = Solver(para, box, bd, pos, np.zeros_like(pos))
-    s.step()
-    gid, cnt, cs, srt, nin = s.grid()
-    assert nin == int(g["n_in_grid"])
-    assert np.array_equal(gid, g["gid"])
-    cells = g["cells"]
-    assert np.array_equal(cs[cells], g["cell_start"])
-    assert np.array_equal(cs[cells + 1], g["cell_end"])
-    for a, b in zip(g["cell_start"], g["cell_end"]):
-        assert list(srt[a:b]) == sorted(g["sorted_ids"][a:b])
-    nbr, ncnt = s.neighbors()
-    rn = g["nbr"]
-    rc = rn[:, 63]
-    assert np.array_equal(ncnt, rc)
-    for i in range(len(ncnt)):
-        assert set(nbr[i, : ncnt[i]]) == set(rn[i, : rc[i]])
-    lam = s.fields()[0]
-    scale = np.abs(g["lam"]).max()
-    np.testing.assert_allclose(lam, g["lam"], rtol=0, atol=2e-5 * scale)
-    # delta-p: the position change of particles the clamp did not touch, less the rounding of the
-    # addition (one ulp of the position)
-    p1, _ = s.get_state()
-    dx = p1.astype(np.float64) - pos.astype(np.float64)
-    inside = ((p1 > bd[:3]) & (p1 < bd[3:])).all(1)
-    ulp = np.spacing(np.abs(pos).astype(np.float32)).astype(np.float64)
-    scale = np.abs(g["offset"]).max()
-    err = np.abs(dx - g["offset"].astype(np.float64)) - ulp
-    assert inside.mean() > 0.5
-    assert (err[inside] <= 5e-5 * scale).all()
+@pytest.mark.parametrize("which", ["32k", "262k"])
+def test_fast_math_one_step_within_1e5(which):
+    """One step from an identical state in the fast mode, three times (from the lattice, then from
+    the oracle's state set into both):
+      * particles at least 2 h from every boundary face: positions within 1e-5 of the oracle's,
+        relative to the largest coordinate, and velocities within that over dt;
+      * every particle: within (iterations + 1) x the clamp jitter amplitude.  The clamp's offset
+        is a hash of the unclamped position (frac(sin(.) * 43758.5453), cuh:112-140), so a
+        last-bit difference before a clamp draws an unrelated offset of up to jitter_coeff, and
+        each iteration clamps again; the difference then spreads about one neighbour layer per
+        iteration, which the 2 h margin covers for 4 iterations (measured: interior 2e-6,
+        1 h - 2 h from a wall 2e-5, nearer up to 6e-4)."""
+    sc = scenes.config_cpu_32k() if which == "32k" else scenes.config_262k()
+    print(f"[case] fast math one step {which}", flush=True)
+    g = Solver.from_scene(sc, flags=FLAG_FAST_MATH)
+    o = Oracle.from_scene(sc)
+    h = float(sc.params["h"])
+    jit = float(sc.params["jitter_coeff"])
+    iters = int(sc.params["iterations"])
+    bd = sc.boundary.astype(np.float64)
+    dt = float(np.float32(sc.params["dt"]))
+    for k in range(3):
+        g.step()
+        o.step(1)
+        gp, gv = g.get_state()
+        op, ov = o.get_state()
+        wall = np.minimum.reduce([op[:, a] - bd[a] for a in range(3)] + [bd[3 + a] - op[:, a] for a in range(3)])
+        inner = wall >= 2.0 * h
+        scale = np.abs(op).max()
+        dpos = np.abs(gp.astype(np.float64) - op).max(1)
+        dvel = np.abs(gv.astype(np.float64) - ov).max(1)
+        print(f"  step {k}: interior ({int(inner.sum())}) positions {dpos[inner].max() / scale:.2e}, "
+              f"velocities {dvel[inner].max():.2e} m/s; all: {dpos.max():.2e} m", flush=True)
+        assert inner.sum() > 0.2 * sc.n
+        assert dpos[inner].max() <= 1e-5 * scale
+        assert dvel[inner].max() <= 1e-5 * scale / dt
+        assert dpos.max() <= (iters + 1) * jit
+        g.set_state(op, ov)  # identical state for the next comparison
+
+
+def fast_stats(sc, steps, every):
+    g = Solver.from_scene(sc, flags=FLAG_DENSITY_STATS | FLAG_FAST_MATH | FLAG_GRAPH)
+    out = []
+    for _ in range(steps // every):
+        g.advance(every)
+        out.append(g.density_stats())
+    g.close()
+    return np.array(out, np.float64)
+
+
+def check_run_statistics(fast, ref, what):
+    """A fast-mode run against the oracle's over the same steps: the run's means of the mean |err|
+    and of the compression within 1%, and of the per-sample maximum error within 5%.  After the
+    first step the two trajectories differ in their last bits and then diverge (the flow is
+    chaotic), so statistics of the run compare, not states or single instants.  The maximum is
+    one particle's and is held to 5%: the oracle itself, restarted with the initial coordinates
+    moved by one ulp, shifted the run mean of the maximum by 0.1-1.9% (and the two means by
+    0.5-0.7%) over 1000 steps of the 32k dam break, three perturbations."""
+    fm, rm = fast.mean(0), ref.mean(0)
+    print(f"  {what}: run means fast {np.round(fm, 5).tolist()} oracle {np.round(rm, 5).tolist()}", flush=True)
+    np.testing.assert_allclose(fm[:2], rm[:2], rtol=1e-2, err_msg=what)
+    np.testing.assert_allclose(fm[2], rm[2], rtol=5e-2, err_msg=what + " (max)")
+
+
+def test_fast_math_1000_steps_32k():
+    """1000 fast-mode steps of the 32k dam break against the oracle's, sampled every 25 steps."""
+    sc = scenes.config_cpu_32k()
+    print("[case] fast math 32k x 1000 steps", flush=True)
+    check_run_statistics(fast_stats(sc, 1000, 25), oracle_stats(sc, 1000, 25)[0], "32k x 1000 steps")
+
+
+@pytest.mark.timeout(900)
+def test_fast_math_1000_steps_262k(oracle_262k_1000):
+    """1000 fast-mode steps of the 262k dam break against the oracle's, sampled every 50 steps."""
+    print("[case] fast math 262k x 1000 steps", flush=True)
+    check_run_statistics(fast_stats(scenes.config_262k(), 1000, 50), oracle_262k_1000[0], "262k x 1000 steps")
