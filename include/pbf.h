@@ -217,10 +217,11 @@ int pbf_profile_step(pbf_solver* s, float* ms, const char** names, int32_t max);
 /* ---- learned warm start and adaptive iteration (no counterpart in the reference) ----------
  * A per-particle MLP 16 -> 64 -> 64 -> 64 -> 1 (SiLU), BF16 weights and activations, FP32
  * accumulation on MFMA, predicts the multiplier a plain step accumulates over its Jacobi
- * iterations (divided by h^2) from 16 features of the particle at its predicted position:
- *   0 C = rho/restRho - 1   1 neighbour count   2 sum_j |grad_j C|^2   3 |grad_i C|
- *   4-6 v dt / h   7 previous step's multiplier / h^2   8 distance to the nearest wall / h
- *   9-14 distance to each wall (x-, x+, y-, y+, z-, z+) / h   15 plain iteration-1 lambda / h^2
+ * iterations 2..I (divided by h^2) from 16 features of the particle after the plain pass 1:
+ *   0 C = rho/restRho - 1   1 neighbour count   2 sum_j |grad_j C|^2 / restRho^2
+ *   3 |grad_i C| / restRho   4-6 v dt / h (predicted velocity)   7 distance to the nearest wall
+ *   / h (<= 4)   8 lambda of pass 2 / h^2   9 lambda of pass 1 / h^2   10 C at the predicted
+ *   positions   11-15 distance to the walls x-, x+, y-, y+, z- / h (<= 4)
  * Parameters are packed FP32: W1[64][16] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64] b4. */
 typedef struct pbf_mlp pbf_mlp;
 int pbf_mlp_param_count(void);

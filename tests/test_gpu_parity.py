@@ -505,6 +505,41 @@ def test_fast_math_one_step_within_1e5(which):
         g.set_state(op, ov)  # identical state for the next comparison
 
 
+@pytest.mark.parametrize("tuning", [(256, 0, True), (64, 0, False), (128, 40, True)])
+def test_fast_math_residual_partials(tuning):
+    """The residuals that steer the adaptive step and the learned gate (workgroup partial sums of
+    max(C, 0) and |C| written by the lambda passes, reduced on the host or on the device) in the
+    fast mode: from the same state each equals the exact mode's within 0.5%, for workgroups of 256
+    and 64 and for a persistent grid of 40 workgroups, and a scattering network is rejected."""
+    from pbf_amd import Mlp
+
+    sc = scenes.config_cpu_32k(seed=12)
+    print(f"[case] fast-mode residual partials, tuning {tuning}", flush=True)
+    res = {}
+    for mode, flags in (("exact", 0), ("fast", FLAG_FAST_MATH)):
+        s = Solver.from_scene(sc, flags=flags)
+        s.set_tuning(*tuning)
+        s.advance(20)
+        p, v = s.get_state()
+        r = [s.step_adaptive(False, 0.0, 1, k)[1] for k in (1, 3)]
+        s.set_state(p, v)
+        m = Mlp(seed=11)  # a network whose pass scatters the fluid: rejected in both modes
+        m.set_params((np.random.default_rng(11).standard_normal(9473) * 3.0).astype(np.float32))
+        m.set_normalization(np.zeros(16), np.ones(16), 1e3, 0.0)
+        s.attach_mlp(m)
+        s.step_adaptive(True, 0.0, 3, 3)
+        info = s.last_guess_info()
+        res[mode] = (r, info)
+        s.close()
+        m.close()
+    (re_, ie), (rf, inf) = res["exact"], res["fast"]
+    print(f"  residuals exact {re_} fast {rf}; gate exact {ie} fast {inf}", flush=True)
+    np.testing.assert_allclose(rf, re_, rtol=5e-3)
+    np.testing.assert_allclose([inf[0], inf[3]], [ie[0], ie[3]], rtol=5e-3)  # after the plain pass 1
+    assert not ie[2] and not inf[2]
+    assert inf[4] > 1.1 * inf[3]
+
+
 def fast_stats(sc, steps, every):
     g = Solver.from_scene(sc, flags=FLAG_DENSITY_STATS | FLAG_FAST_MATH | FLAG_GRAPH)
     out = []
