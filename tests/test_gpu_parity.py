@@ -421,6 +421,22 @@ def test_20_steps_262k_bitexact():
     assert_bitexact(gv, ov, "velocities")
 
 
+@pytest.mark.timeout(600)
+def test_60_steps_1m_bitexact():
+    """The double dam break at 1,048,576 particles and 8 iterations (the largest single scene of
+    the bench), graph-replayed on the GPU, equals the oracle bit for bit at steps 20, 40 and 60
+    (the oracle takes about 25 s for them on 16 host threads)."""
+    print("[case] 1M double dam break x 60 steps", flush=True)
+    g, o = pair(scenes.config_1m(), flags=FLAG_GRAPH)
+    for k in range(3):
+        g.advance(20)
+        o.step(20)
+        gp, gv = g.get_state()
+        op, ov = o.get_state()
+        assert_bitexact(gp, op, f"positions at step {20 * (k + 1)}")
+        assert_bitexact(gv, ov, f"velocities at step {20 * (k + 1)}")
+
+
 def oracle_stats(sc, steps, every):
     """The oracle's density-error statistics (mean |err|, mean max(err, 0), max err) every `every`
     steps over `steps` steps, and its final state."""
