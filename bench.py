@@ -30,7 +30,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from pbf_amd import FLAG_FAST_MATH, FLAG_GRAPH, DeviceBuffer, Solver, learned, lib, scenes  # noqa: E402
+from pbf_amd import FLAG_FAST_MATH, FLAG_GRAPH, DeviceBuffer, Solver, driver, learned, lib, scenes  # noqa: E402
 
 METRIC = "ms/step + particle-steps/s @262k; time to 0.5% density err, learned vs plain"
 HBM_PEAK_GBS = 8000.0
@@ -170,6 +170,34 @@ def learned_leg(n_side, box, train_seeds, eval_seeds, rec_steps, stride, warm, g
                             "avg_launch_ms": ms}
     m.close()
     return summ
+
+
+def reference_app_leg(warmup: int, steps: int, frames: int = 100):
+    """The workload behind the reference's one published number (12-15 ms per frame for 262,144
+    particles on an RTX 3090, its README): the app's scene 4 (src/renderapp.cpp:137-234, the
+    default scene of src/main.cpp) with the app's 13 parameters (h = 1, rest density 6,
+    4 iterations, dt = 0.016, renderapp.cpp:116-129), gravity -49 and the fixed 64^3 grid.
+    Graph-replayed steps as in the headline, and the app's own frame loop: one step per frame
+    with the host clock around the step and its synchronisation (renderapp.cpp:497-502)."""
+    sc = driver.reference_scene(4)
+    s = Solver.from_scene(sc, flags=FLAG_GRAPH)
+    s.advance(warmup)
+    s.synchronize()
+    ms = s.time_steps(steps) / max(steps, 1)
+    t = []
+    for _ in range(frames):
+        t0 = time.perf_counter()
+        s.step()
+        s.synchronize()
+        t.append(time.perf_counter() - t0)
+    s.close()
+    frame_ms = 1000.0 * float(np.median(t))
+    return {"scene": "reference app scene 4 (renderapp.cpp:137-234), 64^3 lattice at h = 1, 64^3 grid",
+            "particles": sc.n, "parameters": {k: (list(v) if isinstance(v, tuple) else v) for k, v in sc.params.items()},
+            "ms_per_step": ms, "value": sc.n / (ms * 1e-3), "unit": "particle-steps/s",
+            "frame_ms_host_clock_median": frame_ms, "frames": frames,
+            "published_ms_per_frame": [12.0, 15.0], "published_hardware": "GeForce RTX 3090 (reference README)",
+            "speedup_vs_published_12ms": 12.0 / ms, "frame_speedup_vs_published_12ms": 12.0 / frame_ms}
 
 
 def copy_bandwidth(mb: int = 512, reps: int = 5) -> float:
@@ -398,6 +426,11 @@ def main():
                                                          (200, 201), 150, 8, 40, 8),
         }
 
+    app = None
+    if args.workload == "dam_break_262k":
+        progress("the reference app's scene 4")
+        app = reference_app_leg(args.warmup, args.steps)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene)
@@ -461,6 +494,7 @@ def main():
         "state_finite": finite,
         "cpu_baseline": cpu,
         "learned": learned_res,
+        "reference_app": app,
         "batch_vs_sequential": seq,
     }
     print(json.dumps(out), flush=True)
