@@ -5,6 +5,7 @@
     built against include/pbf_solver.hpp that sets only the 13 fields of the reference's
     SolverParameter (renderapp.cpp:116-129) and reads positions from its own device buffer
     (exportPositions, the reference's VBO ownership, renderapp.cpp:236-249);
+  * the app's other four scenes at full size, graph-replayed, against the oracle;
   * the 24-scene batch (about 1.74M particles, one launch per kernel for all scenes): every
     scene equals its own oracle;
   * checkpoint / resume through the state file, and the device export.
@@ -116,6 +117,27 @@ def test_reference_app_scene4_through_the_facade():
     assert same_bits(out[1], op), "positions (readState)"
     assert same_bits(out[2], ov), "velocities (readState)"
     assert np.abs(op - sc.pos).max() > 0.1, "the scene moves"
+
+
+@pytest.mark.parametrize("index", [1, 2, 3, 5])
+def test_reference_app_scenes_equal_the_oracle(index):
+    """The reference app's other scenes (src/renderapp.cpp:137-234) at their full 64^3 lattices
+    with the app's parameters and the fixed 64^3 grid: a column thrown up at 20 m/s (1), a block
+    thrown sideways (2), two blocks colliding at 50 m/s (3), a 50 m/s swirl (5).  Ten
+    graph-replayed steps, bit for bit against the oracle; the scenes reach the walls and the
+    grid's edges."""
+    sc = driver.reference_scene(index)
+    print(f"[case] reference app scene {index}, n={sc.n}, 10 steps", flush=True)
+    g = Solver.from_scene(sc, flags=FLAG_GRAPH)
+    g.advance(10)
+    gp, gv = g.get_state()
+    g.close()
+    o = Oracle.from_scene(sc)
+    o.step(10)
+    op, ov = o.get_state()
+    assert same_bits(gp, op), "positions"
+    assert same_bits(gv, ov), "velocities"
+    assert np.abs(op - sc.pos).max() > 0.5, "the scene moves"
 
 
 def test_batch_of_24_scenes_each_equals_its_oracle():
