@@ -295,7 +295,8 @@ int pbf_set_learned(pbf_solver* s, int32_t passes);
  * threads issuing 65,536 wave64 VALU instructions per wavefront, 6: the vorticity pass with the
  * next step's hash, scan, scatter + rank + reorder (the sort stage of a step whose predecessor
  * hashed for it, with the vorticity pass), 7: the vorticity pass alone; the vorticity replays
- * write a spare buffer), replayed reps times back to back on the state of the last step,
+ * write a spare buffer; 8: the curl pass with the XSPH sum), replayed reps times back to back on
+ * the state of the last step,
  * between HIP events on the solver's stream.  The state is unchanged (the replays write only
  * what the next step rebuilds); the diagnostics of the last step (pbf_get_grid,
  * pbf_get_neighbors, pbf_get_fields) no longer describe it afterwards.  No counterpart in the

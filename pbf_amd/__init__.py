@@ -378,7 +378,7 @@ class Solver:
         return float(ms[0])
 
     STAGES = {"neighbors_lambda": 0, "lambda": 1, "delta_apply": 2, "sort_reorder": 3, "launch_floor": 4,
-              "valu_probe": 5, "vort_hash_sort": 6, "vort_apply": 7}
+              "valu_probe": 5, "vort_hash_sort": 6, "vort_apply": 7, "curl": 8}
     VALU_PROBE_INSTR = 2048 * 4 * 16 * 4096  # wave64 VALU instructions of one probe launch (loop body)
 
     def time_stage(self, stage: str, reps: int = 50) -> float:

@@ -1003,14 +1003,16 @@ int pbf_profile_step(pbf_solver* s, float* ms, const char** names, int32_t max) 
 
 // Average duration of one launch of a step stage (0: neighbour build + lambda 1, 1: lambda,
 // 2: delta-p + apply, 3: predict + hash, scan, scatter + rank + reorder, 4: an empty kernel of
-// the particle grid's shape, the floor of one dependent launch, 5: the VALU issue-rate probe),
-// replayed reps times
+// the particle grid's shape, the floor of one dependent launch, 5: the VALU issue-rate probe,
+// 6: the vorticity pass with the next step's hash, then the sort stage, 7: the vorticity pass,
+// 8: the curl pass with the XSPH sum), replayed reps times
 // back to back on the state of the last step between two HIP events on the solver's stream.
 // The replays write only buffers that the next step rebuilds (cell tables, lists, lambda in
 // pos.w, the free position / velocity / id buffers, x_old; the scan clears the cell counters
-// again), so the state is unchanged.
+// again; the curl pass rewrites omega, |omega| in pos.w and the XSPH sums in x_old with the
+// values the step left there), so the state is unchanged.
 int pbf_time_stage(pbf_solver* s, int32_t stage, int32_t reps, float* ms) {
-    if (!s || !ms || reps < 1 || stage < 0 || stage > 7) return set_error(PBF_ERR_ARG, "bad argument");
+    if (!s || !ms || reps < 1 || stage < 0 || stage > 8) return set_error(PBF_ERR_ARG, "bad argument");
     if (s->steps_done < 1 || s->n == 0) return set_error(PBF_ERR_STATE, "run a step first");
     HIP_TRY(hipSetDevice(s->device));
     const StepConsts K = make_consts(s);
@@ -1043,6 +1045,8 @@ int pbf_time_stage(pbf_solver* s, int32_t stage, int32_t reps, float* ms) {
                                 B.vel[s->vc ^ 1], B.ids[s->ic ^ 1], nullptr);
         }
         if (stage == 7) launch_vort_apply(s->stream, K, s->geo, s->tune, B.pos[a], B.vel[s->vc ^ 1], B.vp, B, false);
+        // the curl pass reads the velocity before the vorticity pass, as the step does
+        if (stage == 8) launch_curl(s->stream, K, s->tune, B.pos[a], B.vel[s->vc ^ 1], B, nullptr, nullptr);
     }
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamEndCapture(s->stream, &g));
