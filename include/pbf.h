@@ -205,7 +205,10 @@ int pbf_time_steps(pbf_solver* s, int32_t nsteps, float* ms_total);
 
 /* Per-stage times of one step (HIP events between stages, one step, no graph):
  * out[0] predict+hash, [1] scan, [2] scatter+rank+reorder, [3] neighbours+lambda1,
- * [4] Jacobi iterations, [5] curl, [6] vorticity+viscosity.  ms. */
+ * [4] Jacobi iterations, [5] curl, [6] vorticity+viscosity.  ms.  A step's vorticity pass also
+ * predicts, hashes and counts for the next step, so out[0] is about 0 and out[6] includes that
+ * hash, unless the state changed since the last step (pbf_set_state, dt, parameters, row order),
+ * which makes the step hash for itself in out[0]. */
 int pbf_stage_times(pbf_solver* s, float out[7]);
 
 /* One step launched eagerly with a HIP event recorded on the solver's stream after every

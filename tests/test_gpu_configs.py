@@ -204,6 +204,53 @@ def test_export_device_equals_get_state():
         buf.close()
 
 
+def test_device_view_is_the_sorted_state():
+    """pbf_device_state: the device pointers of the cell-sorted state (the reference's SolverData
+    buffers, include/solverBase.h:45-47).  Slot k holds particle ids[k]: its position and
+    velocity equal pbf_get_state's row ids[k] bit for bit, ids is a permutation, w of the
+    velocities is 0 and w of the positions is the step's |curl|."""
+    sc = scenes.config_cpu_32k(seed=14)
+    print(f"[case] device view {sc.name}", flush=True)
+    s = Solver.from_scene(sc, flags=FLAG_GRAPH | pbf_amd.FLAG_DEBUG)
+    s.advance(5)
+    s.synchronize()
+    v = s.device_view()
+    assert v.n == sc.n and v.ncells >= int(np.prod(sc.grid_dims))
+    p4 = np.empty((sc.n, 4), np.float32)
+    v4 = np.empty((sc.n, 4), np.float32)
+    ids = np.empty(sc.n, np.uint32)
+    L = pbf_amd.lib()
+    for dst, src in ((p4, v.pos4), (v4, v.vel4), (ids, v.ids)):
+        assert L.pbf_memcpy_to_host(dst.ctypes.data, src, dst.nbytes) == 0
+    gp, gv = s.get_state()
+    assert np.array_equal(np.sort(ids), np.arange(sc.n, dtype=np.uint32))
+    assert same_bits(p4[:, :3], gp[ids]) and same_bits(v4[:, :3], gv[ids])
+    assert not v4[:, 3].any()
+    curl = s.fields()[3][ids].astype(np.float32)
+    mag = np.sqrt((curl[:, 0] * curl[:, 0] + curl[:, 1] * curl[:, 1]) + curl[:, 2] * curl[:, 2])
+    np.testing.assert_allclose(p4[:, 3], mag, rtol=1e-6, atol=0)
+    s.close()
+
+
+def test_stage_times_show_the_fused_hash():
+    """pbf_stage_times: seven stage times of one step.  After a step, the next one's predict and
+    hash ran in its vorticity pass, so the first stage holds only the events' own interval; after
+    pbf_set_state the step hashes for itself and that stage takes longer by the hash kernel."""
+    sc = scenes.config_262k(seed=15)
+    print(f"[case] stage times {sc.name}", flush=True)
+    s = Solver.from_scene(sc)
+    s.advance(3)
+    t = s.stage_times()
+    assert (t >= 0).all() and t[3] > 0.01 and t[4] > 0.05
+    p, v = s.get_state()
+    s.set_state(p, v)
+    u = s.stage_times()
+    print(f"  after a step: {np.round(t, 4).tolist()}; after set_state: {np.round(u, 4).tolist()}", flush=True)
+    # two events with nothing between them already read ~5 us apart; the hash adds ~4 us at 262k
+    assert u[0] > t[0] + 0.0015
+    s.close()
+
+
 def test_graphs_survive_set_state():
     """pbf_set_state keeps the captured step graphs: steps after an upload replay them and equal
     eager steps from the same upload."""
