@@ -121,7 +121,11 @@ int pbf_autotune(pbf_solver* s, int32_t steps, int32_t* best, float* best_ms);
 int pbf_solve(pbf_solver* s, const pbf_params* para);
 
 /* One step with the solver's current parameters and time step dt (para.dt = dt,
- * para.invDt = 1/dt), as cudaPBFSolve (kernels/cudaPBF.cu:50) called with an updated para.dt. */
+ * para.invDt = 1/dt), as cudaPBFSolve (kernels/cudaPBF.cu:50) called with an updated para.dt.
+ * With PBF_FLAG_GRAPH, a pbf_solve / pbf_step call whose parameters differ from the last ones
+ * recaptures the step's graph; from the second such call in a row on, steps launch eagerly (no
+ * capture) until a call leaves the parameters unchanged, so a caller that adapts dt every step
+ * pays no capture per step.  Results are the same either way. */
 int pbf_step(pbf_solver* s, float dt);
 
 /* nsteps steps with the current parameters: the app's frame loop of cudaPBFSolve calls

@@ -81,6 +81,9 @@ struct pbf_solver {
     // with the counters zeroed, by anything that changes the state, the step parameters or the
     // cell order (invalidate_hash).
     bool hash_ready = false;
+    // consecutive pbf_solve / pbf_step calls that changed the parameters: from the second on, steps
+    // launch eagerly (no capture, no drain of the stream) until a call leaves them unchanged
+    int param_streak = 0;
     // warm start: features, accumulated multiplier, residual partials, attached network
     float* feat = nullptr;
     float* lam_sum = nullptr;
@@ -422,9 +425,11 @@ void advance_parity(pbf_solver* s, int k) {
 bool uses_graphs(const pbf_solver* s) { return (s->P.flags & PBF_FLAG_GRAPH) && !(s->P.flags & PBF_FLAG_DEBUG); }
 
 // k steps (1 or kGraphSteps): eagerly, or from the graph captured for (kind of step, k, parity).
+// Parameters that change on every call (a caller adapting dt each step) would recapture a graph
+// per step; those steps launch eagerly instead.
 int run_steps(pbf_solver* s, int k) {
     HIP_TRY(hipSetDevice(s->device));
-    if (!uses_graphs(s)) {
+    if (!uses_graphs(s) || s->param_streak >= 2) {
         for (int j = 0; j < k; ++j)
             if (int rc = enqueue_step(s, nullptr)) return rc;
         return PBF_OK;
@@ -438,8 +443,10 @@ int run_steps(pbf_solver* s, int k) {
 
 int run_step(pbf_solver* s) { return run_steps(s, 1); }
 
-// nsteps steps: graphs of kGraphSteps steps while that many remain, then single steps.
+// nsteps steps: graphs of kGraphSteps steps while that many remain, then single steps.  The
+// parameters stay as they are for all of them.
 int run_many(pbf_solver* s, int nsteps) {
+    s->param_streak = 0;
     int i = 0;
     for (; i + kGraphSteps <= nsteps; i += kGraphSteps)
         if (int rc = run_steps(s, kGraphSteps)) return rc;
@@ -760,6 +767,9 @@ int pbf_solve(pbf_solver* s, const pbf_params* para) {
         invalidate_hash(s);  // the last step predicted with the old parameters
         s->P = *para;
         drop_graphs(s);
+        s->param_streak++;
+    } else {
+        s->param_streak = 0;
     }
     return run_step(s);
 }
@@ -772,6 +782,9 @@ int pbf_step(pbf_solver* s, float dt) {
         s->P.dt = dt;
         s->P.invDt = 1.0f / dt;
         drop_graphs(s);
+        s->param_streak++;
+    } else {
+        s->param_streak = 0;
     }
     return run_step(s);
 }

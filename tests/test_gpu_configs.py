@@ -274,6 +274,30 @@ def test_graphs_survive_set_state():
     e.close()
 
 
+def test_dt_changing_every_step_matches_eager():
+    """A caller that adapts dt on every call (pbf_step): with PBF_FLAG_GRAPH the steps from the
+    second change in a row on launch eagerly, then graphs again once dt holds; every step equals
+    an eager solver's bit for bit, and so do the graph-replayed steps of a following advance."""
+    sc = scenes.config_cpu_32k(seed=16)
+    print("[case] dt changing every step", flush=True)
+    g = Solver.from_scene(sc, flags=FLAG_GRAPH)
+    e = Solver.from_scene(sc)
+    dts = [0.008, 0.0075, 0.007, 0.0072, 0.0072, 0.0072, 0.0068, 0.0081, 0.0081]
+    for k, dt in enumerate(dts):
+        g.step(dt)
+        e.step(dt)
+        gp, gv = g.get_state()
+        ep, ev = e.get_state()
+        assert same_bits(gp, ep) and same_bits(gv, ev), f"step {k} (dt {dt})"
+    g.advance(9)
+    e.advance(9)
+    gp, gv = g.get_state()
+    ep, ev = e.get_state()
+    assert same_bits(gp, ep) and same_bits(gv, ev)
+    g.close()
+    e.close()
+
+
 @pytest.mark.parametrize("change", ["set_state", "dt", "ordering", "parameters"])
 def test_changes_between_steps_rehash(change):
     """A step predicts, hashes and counts the next step's cells with the step's own parameters,
