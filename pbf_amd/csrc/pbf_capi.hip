@@ -536,11 +536,12 @@ int create_solver(const pbf_params* para, const float* grid_space, const float* 
     int rc;
     // pos / vel hold the padding slot n after the particles (see StepBuffers)
     const float4 far_pos = make_float4(kFarPos, kFarPos, kFarPos, 0.0f), still = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const std::vector<float4> far_tail((size_t)kSearchTail, far_pos);
     for (int i = 0; i < 2; ++i) {
-        if ((rc = dalloc(s, &B.pos[i], np + 1))) return fail(rc);
+        if ((rc = dalloc(s, &B.pos[i], np + kSearchTail))) return fail(rc);
         if ((rc = dalloc(s, &B.vel[i], np + 1))) return fail(rc);
         if ((rc = dalloc(s, &B.ids[i], np))) return fail(rc);
-        if (hipMemcpy(B.pos[i] + n, &far_pos, sizeof(float4), hipMemcpyHostToDevice) != hipSuccess ||
+        if (hipMemcpy(B.pos[i] + n, far_tail.data(), sizeof(float4) * kSearchTail, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(B.vel[i] + n, &still, sizeof(float4), hipMemcpyHostToDevice) != hipSuccess)
             return fail(set_error(PBF_ERR_HIP, "padding slot upload failed"));
     }

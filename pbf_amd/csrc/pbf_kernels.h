@@ -11,7 +11,8 @@ namespace pbf {
 // Device buffers of one solver; index [2] arrays alternate between steps / iterations.
 // pos[i] and vel[i] hold n + 1 entries: slot n is a padding particle at (kFarPos, kFarPos,
 // kFarPos) with zero velocity and multiplier, which no kernel writes.  The neighbour list pads
-// each particle's last group of 8 entries with n.
+// each particle's last group of 8 entries with n.  pos[i] holds kSearchTail such padding slots
+// (n .. n + kSearchTail - 1): the search's candidate loads run up to 31 slots past a run's end.
 struct StepBuffers {
     float4* pos[2];      // .w: lambda during the constraint passes, |omega| after the curl pass
     float4* vel[2];
@@ -48,6 +49,7 @@ constexpr int kNumFeatures = 16;
 // neighbour sum is +0 or -0.  Sums start at +0 and never become -0, so adding such a term leaves
 // them unchanged bit for bit: a padded group is evaluated as if its tail were absent.
 constexpr float kFarPos = 1e18f;
+constexpr int kSearchTail = 32;
 
 // Particles per solver (all scenes of a batch) stay below 2^28, so the byte offset of any
 // float4 slot, padding slot included, fits in 32 bits (slot_ref).

@@ -453,6 +453,8 @@ __device__ void block_sum2(float c, float d, float* __restrict__ partial) {
 // from the particle cannot hold a neighbour and is left out of the run (the x-cells that remain
 // are contiguous, so the order is unchanged).  Distances are taken to the particle's own cell
 // faces less a margin of 1e-3 h, which covers the rounding of the cell index.
+// The list is assembled in LDS (one row per particle) and then stored one group of 8 at a time,
+// 32 contiguous bytes per subgroup, the last group padded with the padding slot n.
 constexpr int kSub = 8;  // lanes per particle in the search
 
 // b[R] = the value of lane R of this lane's group of 8 (R = 0..RMAX), for two values: a bit-mask
@@ -464,6 +466,14 @@ __device__ __forceinline__ void subgroup_bcast(uint32_t x, uint32_t y, uint32_t*
     if constexpr (RMAX > 0) subgroup_bcast<RMAX - 1>(x, y, bx, by);
 }
 
+// Words per particle row of the search's LDS list: the cap rounded up to 8, plus one so that the
+// rows of a wavefront's eight particles start in different banks, when the workgroup's rows still
+// fit in 64 KiB.
+__host__ __device__ constexpr int search_lds_row(int cap, int ppb) {
+    const int r8 = (cap + 7) & ~7;
+    return ppb * (r8 + 1) * 4 <= 65536 ? r8 + 1 : r8;
+}
+
 // WIDE: the table's group stride nstride * 8 needs more than 24 bits (n > 2^21), so list
 // offsets take a full 32-bit multiply instead of the 24-bit one.
 template <int BS, bool WIDE>
@@ -471,11 +481,14 @@ __global__ __launch_bounds__(BS) void k_search(StepConsts K, const SceneGeom* __
                                                const uint8_t* __restrict__ sid, const float4* __restrict__ pos,
                                                const uint32_t* __restrict__ cstart, uint32_t* __restrict__ nbr,
                                                uint8_t* __restrict__ ncnt, int xcd_map) {
+    extern __shared__ uint32_t lds_list[];
     constexpr int PPB = BS / kSub;  // particles per workgroup pass
+    const int row_words = search_lds_row(K.cap, PPB);
     const BlockRange R((K.n + PPB - 1) / PPB, xcd_map);
     const int lane8 = threadIdx.x & (kSub - 1);
     const int wsub = (threadIdx.x & 63) / kSub;  // subgroup inside the wavefront
     const uint32_t below = (1u << lane8) - 1u;
+    uint32_t* const lrow = lds_list + (threadIdx.x / kSub) * row_words;
     for (int blk = R.first; blk < R.last; ++blk) {
         const int s = blk * PPB + (int)threadIdx.x / kSub;
         if (s >= K.n) continue;  // whole subgroups leave together
@@ -528,49 +541,53 @@ __global__ __launch_bounds__(BS) void k_search(StepConsts K, const SceneGeom* __
         run_bounds(lane8, mb, me);
         run_bounds(8, rb[8], re[8]);
         subgroup_bcast<7>(mb, me, rb, re);
-        // Four chunks of 8 candidates per pass: their loads are issued together.  A chunk that
-        // holds no candidate for any particle of the wavefront is skipped (a wave-uniform
-        // branch), so a run costs ceil(len / 8) chunks, not a multiple of four.
+        // Four chunks of 8 candidates per pass: their loads are issued together, at immediate
+        // offsets from one address.  Loads past the run's end (at most 31 slots, since the pass
+        // starts inside the run) read particles of later runs or the padding slots after slot n;
+        // their lanes are masked out.  A chunk that holds no candidate for any particle of the
+        // wavefront is skipped (a wave-uniform branch), so a run costs ceil(len / 8) chunks.
         constexpr int kChunks = 4;
-        uint32_t* const lbase = nbr + (size_t)s * 8;  // entry j at lbase[(j / 8) * gstride + j % 8]
-        const uint32_t gstride = (uint32_t)K.nstride * 8u;
+        const char* const pb = reinterpret_cast<const char*>(pos);
         int cnt = 0;
 #pragma unroll
         for (int r = 0; r < 9; ++r) {
-            const uint32_t e = re[r];
-            for (uint32_t q0 = rb[r]; q0 < e && cnt < K.cap; q0 += kChunks * kSub) {
+            const int e = (int)re[r];
+            for (int q0 = (int)rb[r]; q0 < e && cnt < K.cap; q0 += kChunks * kSub) {
+                const int ql = q0 + lane8;  // this lane's candidate in chunk 0
+                const int left = e - ql;    // chunk u holds a candidate for this lane iff left > u * kSub
+                const char* const pl = pb + (size_t)((uint32_t)ql << 4);
                 float4 xs[kChunks];
 #pragma unroll
-                for (int u = 0; u < kChunks; ++u) xs[u] = slot_ref(pos, min(q0 + (uint32_t)(u * kSub + lane8), e - 1u));
+                for (int u = 0; u < kChunks; ++u) xs[u] = *reinterpret_cast<const float4*>(pl + u * kSub * 16);
 #pragma unroll
                 for (int u = 0; u < kChunks; ++u) {
-                    const uint32_t q = q0 + (uint32_t)(u * kSub + lane8);
-                    // wave masks of the three compares (predicates: 36 unsigned <, 33 !=, 5 ordered
-                    // <=), combined in scalar registers; the run mask also decides the skip
-                    const bool in_run = q < e;
+                    // wave masks of the compares (predicates: 5 ordered <=), combined in scalar
+                    // registers; the run mask also decides the skip
+                    const bool in_run = left > u * kSub;
                     const uint64_t m_run = __builtin_amdgcn_ballot_w64(in_run);
                     if (u > 0 && m_run == 0) break;
                     const float rx = xi.x - xs[u].x, ry = xi.y - xs[u].y, rz = xi.z - xs[u].z;
                     const float d2 = rx * rx + ry * ry + rz * rz;
-                    const bool other = q != (uint32_t)s, close = d2 <= K.h2;
-                    const uint64_t m = m_run & __builtin_amdgcn_ballot_w64(other) & __builtin_amdgcn_fcmpf(d2, K.h2, 5);
-                    const bool hit = in_run && other && close;
+                    // the particle itself is a candidate of its own row's run only (r = 4: dz = dy = 0)
+                    const bool other = r != 4 || ql + u * kSub != s;
+                    uint64_t m = m_run & __builtin_amdgcn_fcmpf(d2, K.h2, 5);
+                    if (r == 4) m &= __builtin_amdgcn_ballot_w64(other);
+                    const bool hit = in_run && other && d2 <= K.h2;
                     const uint32_t bits = (uint32_t)(m >> (kSub * wsub)) & 0xffu;
                     const int at = cnt + __popc(bits & below);
-                    if (hit && at < K.cap) {
-                        const uint32_t g = (uint32_t)(at >> 3);
-                        lbase[(WIDE ? g * gstride : __umul24(g, gstride)) + (uint32_t)(at & 7)] = q;
-                    }
+                    if (hit && at < K.cap) lrow[at] = (uint32_t)(ql + u * kSub);
                     cnt += __popc(bits);
                 }
             }
         }
-        // pad the last group of 8 with the padding slot n
+        // the list to its groups of 8; the lanes of a subgroup read their own particle's row
         const int c = min(cnt, K.cap);
-        const int pe = c + lane8;
-        if (pe < ((c + 7) & ~7)) {
-            const uint32_t g = (uint32_t)(pe >> 3);
-            lbase[(WIDE ? g * gstride : __umul24(g, gstride)) + (uint32_t)(pe & 7)] = (uint32_t)K.n;
+        uint32_t* const lbase = nbr + (size_t)s * 8;  // entry j at lbase[(j / 8) * gstride + j % 8]
+        const uint32_t gstride = (uint32_t)K.nstride * 8u;
+        for (int g = 0; g * 8 < c; ++g) {
+            const int j = g * 8 + lane8;
+            const uint32_t v = j < c ? lrow[j] : (uint32_t)K.n;
+            lbase[(WIDE ? (uint32_t)g * gstride : __umul24((uint32_t)g, gstride)) + (uint32_t)lane8] = v;
         }
         if (lane8 == 0) ncnt[s] = (uint8_t)c;
     }
@@ -1001,6 +1018,11 @@ static inline unsigned ngrid_search(const Tuning& T, int n) {
     return T.grid > 0 ? std::min<unsigned>((unsigned)T.grid, nb) : nb;
 }
 
+// Dynamic LDS of a search workgroup of bs lanes: one list row per particle.
+static inline size_t search_lds_bytes(const StepConsts& K, int bs) {
+    return (size_t)(bs / kSub) * (size_t)search_lds_row(K.cap, bs / kSub) * sizeof(uint32_t);
+}
+
 void launch_predict_hash(hipStream_t st, const StepConsts& K, const Geometry& GE, const float4* pos, const float4* vel,
                          const StepBuffers& B) {
     if (K.n > 0)
@@ -1062,10 +1084,10 @@ void launch_neighbors_lambda(hipStream_t st, const StepConsts& K, const Geometry
                              const StepBuffers& B, float* dbg_lam, const LearnArgs& L) {
     if (K.n <= 0) return;
     if ((int64_t)K.nstride * 8 >= (int64_t)1 << 24) {
-        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_search<BS, true>), dim3(ngrid_search(T, K.n)), dim3(BS), 0, st, K,
+        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_search<BS, true>), dim3(ngrid_search(T, K.n)), dim3(BS), search_lds_bytes(K, BS), st, K,
                                               GE.geoms, GE.sid, pos, B.cstart, B.nbr, B.ncnt, T.xcd_map));
     } else {
-        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_search<BS, false>), dim3(ngrid_search(T, K.n)), dim3(BS), 0, st, K,
+        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_search<BS, false>), dim3(ngrid_search(T, K.n)), dim3(BS), search_lds_bytes(K, BS), st, K,
                                               GE.geoms, GE.sid, pos, B.cstart, B.nbr, B.ncnt, T.xcd_map));
     }
     PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_lambda_first<BS, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, B.nbr,
