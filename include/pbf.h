@@ -73,7 +73,7 @@ typedef struct pbf_params {
 typedef struct pbf_solver pbf_solver;
 
 /* Replaces initCUDASolverDataBuffer (kernels/cudaPBF.h:8-10, kernels/cudaPBF.cu:4-30):
- * allocates every device buffer for n particles (0 <= n < 2^28) and uploads positions and
+ * allocates every device buffer for n particles (0 <= n < 2^28 - 32) and uploads positions and
  * velocities.  grid_space = {pmin xyz, pmax xyz}: only pmin anchors the grid (cuh:156-160), as in the
  * reference where the app passes the box (src/renderapp.cpp:251).  boundary = the clamp box
  * (box shrunk by the particle radius, renderapp.cpp:131).  vel_xyz may be NULL (zero). */

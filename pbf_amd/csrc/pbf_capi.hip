@@ -599,7 +599,7 @@ int pbf_create(const pbf_params* para, const float grid_space[6], const float bo
     int rc = check_params(para);
     if (rc) return rc;
     if (n < 0) return set_error(PBF_ERR_ARG, "n < 0");
-    if (n >= kMaxSlots) return set_error(PBF_ERR_ARG, "n must be below 2^28");
+    if (n >= kMaxSlots) return set_error(PBF_ERR_ARG, "n must be below 2^28 - 32");
     if (n > 0 && !pos_xyz) return set_error(PBF_ERR_ARG, "pos_xyz is NULL");
     if (!grid_space || !boundary) return set_error(PBF_ERR_ARG, "grid_space / boundary is NULL");
     int ncells = para->grid_dims[0] * para->grid_dims[1] * para->grid_dims[2];
@@ -895,7 +895,7 @@ int pbf_get_neighbors(pbf_solver* s, int32_t* nbr, int32_t* cnt) {
         if (nbr)
             for (int j = 0; j < s->cap; ++j)
                 nbr[(size_t)id * s->cap + j] =
-                    j < c[sl] ? (int32_t)ids[tab[((size_t)(j / 8) * s->nstride + sl) * 8 + (j % 8)]] : -1;
+                    j < c[sl] ? (int32_t)ids[tab[((size_t)(j / 8) * s->nstride + sl) * 8 + (j % 8)] >> 4] : -1;
     }
     return PBF_OK;
 }

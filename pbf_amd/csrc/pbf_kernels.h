@@ -28,8 +28,9 @@ struct StepBuffers {
     uint32_t* tile_sum;  // scan tiles
     uint32_t* bucket;
     uint32_t* bucket_id; // particle id of each bucket entry (ranking inside a cell)
-    uint32_t* nbr;       // ceil(cap / 8) * 8 * nstride, blocks of 8 entries per slot; entries
-                         // cnt .. ceil8(cnt) - 1 of a slot hold the padding slot n
+    uint32_t* nbr;       // ceil(cap / 8) * 8 * nstride, blocks of 8 entries per slot, each the
+                         // neighbour's byte offset 16 q in the float4 slot arrays; entries
+                         // cnt .. ceil8(cnt) - 1 of a slot hold the padding slot n (16 n)
     uint8_t* ncnt;
     float* lam1;         // iteration-1 lambda per slot (a warm-start feature)
 };
@@ -51,9 +52,9 @@ constexpr int kNumFeatures = 16;
 constexpr float kFarPos = 1e18f;
 constexpr int kSearchTail = 32;
 
-// Particles per solver (all scenes of a batch) stay below 2^28, so the byte offset of any
-// float4 slot, padding slot included, fits in 32 bits (slot_ref).
-constexpr int64_t kMaxSlots = (int64_t)1 << 28;
+// Particles per solver (all scenes of a batch) stay below 2^28 - kSearchTail, so the byte offset
+// of any float4 slot, padding slots included, fits in 32 bits (slot_at, neighbour-list entries).
+constexpr int64_t kMaxSlots = ((int64_t)1 << 28) - kSearchTail;
 
 // Optional outputs of the constraint kernels (all NULL on the plain path).
 struct LearnArgs {

@@ -7,7 +7,8 @@
 //                                     multiplier accumulated last step (warm start only)
 //   xold4  float4 (x_old)             position at the start of the step
 //   omega4 float4 (wx, wy, wz, |w|)   curl
-//   nbr    uint32 blocks of 8          neighbour slots: entry j of slot s at
+//   nbr    uint32 blocks of 8          neighbour slots as byte offsets into the float4 slot
+//                                     arrays (slot q stored as 16 q): entry j of slot s at
 //                                     ((j / 8) * nstride + s) * 8 + j % 8, so a particle's group
 //                                     of 8 is 32 contiguous bytes and a wavefront's group is one
 //                                     contiguous 2 KB range
@@ -386,10 +387,10 @@ __device__ __forceinline__ size_t nbr_index(int s, int j, int nstride) {
 // so a group is straight-line code with no per-entry test.
 constexpr int kGroup = 8;
 
-// Entry q of a float4 slot array by a 32-bit byte offset (q <= n < 2^28): the load takes the
-// array's scalar base and one vector offset.
-__device__ __forceinline__ const float4& slot_ref(const float4* __restrict__ base, uint32_t q) {
-    return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + (q << 4));
+// Entry q of a float4 slot array by its 32-bit byte offset qb = 16 q (q <= n + 31 < 2^28), the value
+// a neighbour-list entry holds: the load takes the array's scalar base and one vector offset.
+__device__ __forceinline__ const float4& slot_at(const float4* __restrict__ base, uint32_t qb) {
+    return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + qb);
 }
 
 // The 8 list entries of group j0 (a multiple of 8, below cnt) of slot s.
@@ -407,8 +408,8 @@ __device__ __forceinline__ void load_group(const uint32_t* __restrict__ nbr, int
     q[7] = b.w;
 }
 
-// Calls f(q, x_q) for the list entries j = 0 .. ceil8(cnt) - 1 of slot s in order, x_q = pos[q]
-// (entries past cnt are the padding slot).
+// Calls f(qb, x_q) for the list entries j = 0 .. ceil8(cnt) - 1 of slot s in order, qb = 16 q the
+// entry's byte offset and x_q = pos[q] (entries past cnt are the padding slot).
 
 template <class F>
 __device__ __forceinline__ void for_neighbors(const uint32_t* __restrict__ nbr, int s, int nstride, int cnt,
@@ -418,7 +419,7 @@ __device__ __forceinline__ void for_neighbors(const uint32_t* __restrict__ nbr, 
         load_group(nbr, s, j0, nstride, q);
         float4 x[kGroup];
 #pragma unroll
-        for (int u = 0; u < kGroup; ++u) x[u] = slot_ref(pos, q[u]);
+        for (int u = 0; u < kGroup; ++u) x[u] = slot_at(pos, q[u]);
 #pragma unroll
         for (int u = 0; u < kGroup; ++u) f(q[u], x[u]);
     }
@@ -466,17 +467,23 @@ __device__ __forceinline__ void subgroup_bcast(uint32_t x, uint32_t y, uint32_t*
     if constexpr (RMAX > 0) subgroup_bcast<RMAX - 1>(x, y, bx, by);
 }
 
-// Words per particle row of the search's LDS list: the cap rounded up to 8, plus one so that the
-// rows of a wavefront's eight particles start in different banks, when the workgroup's rows still
-// fit in 64 KiB.
+// Words per particle row of the search's LDS list.  A pass of 4 chunks starts with fewer than cap
+// hits and adds at most 32, so a row of cap + 32 words or more (odd, so that the rows of a
+// wavefront's eight particles start in different banks) takes every hit without a test against
+// the cap; the hits past the cap are never read.  When such rows do not fit in 64 KiB, rows of
+// cap words and a test per hit.
+__host__ __device__ constexpr int search_lds_row_slack(int cap) { return ((cap + 7) & ~7) + 33; }
+__host__ __device__ constexpr bool search_lds_slack(int cap, int ppb) {
+    return ppb * search_lds_row_slack(cap) * 4 <= 65536;
+}
 __host__ __device__ constexpr int search_lds_row(int cap, int ppb) {
-    const int r8 = (cap + 7) & ~7;
-    return ppb * (r8 + 1) * 4 <= 65536 ? r8 + 1 : r8;
+    return search_lds_slack(cap, ppb) ? search_lds_row_slack(cap) : ((cap + 7) & ~7);
 }
 
 // WIDE: the table's group stride nstride * 8 needs more than 24 bits (n > 2^21), so list
-// offsets take a full 32-bit multiply instead of the 24-bit one.
-template <int BS, bool WIDE>
+// offsets take a full 32-bit multiply instead of the 24-bit one.  SLACK: LDS rows with room for
+// the hits past the cap (search_lds_row).
+template <int BS, bool WIDE, bool SLACK>
 __global__ __launch_bounds__(BS) void k_search(StepConsts K, const SceneGeom* __restrict__ geoms,
                                                const uint8_t* __restrict__ sid, const float4* __restrict__ pos,
                                                const uint32_t* __restrict__ cstart, uint32_t* __restrict__ nbr,
@@ -575,7 +582,7 @@ __global__ __launch_bounds__(BS) void k_search(StepConsts K, const SceneGeom* __
                     const bool hit = in_run && other && d2 <= K.h2;
                     const uint32_t bits = (uint32_t)(m >> (kSub * wsub)) & 0xffu;
                     const int at = cnt + __popc(bits & below);
-                    if (hit && at < K.cap) lrow[at] = (uint32_t)(ql + u * kSub);
+                    if (hit && (SLACK || at < K.cap)) lrow[at] = ((uint32_t)ql << 4) + (uint32_t)(u * kSub * 16);
                     cnt += __popc(bits);
                 }
             }
@@ -586,7 +593,7 @@ __global__ __launch_bounds__(BS) void k_search(StepConsts K, const SceneGeom* __
         const uint32_t gstride = (uint32_t)K.nstride * 8u;
         for (int g = 0; g * 8 < c; ++g) {
             const int j = g * 8 + lane8;
-            const uint32_t v = j < c ? lrow[j] : (uint32_t)K.n;
+            const uint32_t v = j < c ? lrow[j] : (uint32_t)K.n << 4;
             lbase[(WIDE ? (uint32_t)g * gstride : __umul24((uint32_t)g, gstride)) + (uint32_t)lane8] = v;
         }
         if (lane8 == 0) ncnt[s] = (uint8_t)c;
@@ -717,7 +724,7 @@ __global__ __launch_bounds__(BS) void k_delta_apply(StepConsts K, const SceneGeo
             float r2 = dot3_t<FAST>(rx, ry, rz);
             float w = poly6(r2, K);
             float t = spiky_scale_t<FAST>(r2, K);
-            const float lam_j = LS ? LA.lam_src[q] : xj.w;
+            const float lam_j = LS ? LA.lam_src[q >> 4] : xj.w;
             float a = (lam_i + lam_j) + (SN == 4 ? scorr4(w, K) : scorr(w, K));
             ax = madd<FAST>(a, rx * t, ax);
             ay = madd<FAST>(a, ry * t, ay);
@@ -765,8 +772,8 @@ __global__ __launch_bounds__(BS) void k_curl(StepConsts K, float4* __restrict__ 
             load_group(nbr, s, j0, K.nstride, qs);
 #pragma unroll
             for (int u = 0; u < kGroup; ++u) {
-                xs[u] = slot_ref(pos, qs[u]);
-                vs[u] = slot_ref(vel, qs[u]);
+                xs[u] = slot_at(pos, qs[u]);
+                vs[u] = slot_at(vel, qs[u]);
             }
 #pragma unroll
             for (int u = 0; u < kGroup; ++u) {
@@ -1083,13 +1090,26 @@ void launch_sort_reorder(hipStream_t st, const StepConsts& K, const Geometry& GE
 void launch_neighbors_lambda(hipStream_t st, const StepConsts& K, const Geometry& GE, const Tuning& T, float4* pos,
                              const StepBuffers& B, float* dbg_lam, const LearnArgs& L) {
     if (K.n <= 0) return;
-    if ((int64_t)K.nstride * 8 >= (int64_t)1 << 24) {
-        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_search<BS, true>), dim3(ngrid_search(T, K.n)), dim3(BS), search_lds_bytes(K, BS), st, K,
-                                              GE.geoms, GE.sid, pos, B.cstart, B.nbr, B.ncnt, T.xcd_map));
+    const bool wide = (int64_t)K.nstride * 8 >= (int64_t)1 << 24;
+    const bool slack = search_lds_slack(K.cap, T.block / kSub);
+#define PBF_SEARCH(W, SL)                                                                                              \
+    PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_search<BS, W, SL>), dim3(ngrid_search(T, K.n)), dim3(BS),                 \
+                                          search_lds_bytes(K, BS), st, K, GE.geoms, GE.sid, pos, B.cstart, B.nbr, B.ncnt, \
+                                          T.xcd_map))
+    if (wide) {
+        if (slack) {
+            PBF_SEARCH(true, true);
+        } else {
+            PBF_SEARCH(true, false);
+        }
     } else {
-        PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_search<BS, false>), dim3(ngrid_search(T, K.n)), dim3(BS), search_lds_bytes(K, BS), st, K,
-                                              GE.geoms, GE.sid, pos, B.cstart, B.nbr, B.ncnt, T.xcd_map));
+        if (slack) {
+            PBF_SEARCH(false, true);
+        } else {
+            PBF_SEARCH(false, false);
+        }
     }
+#undef PBF_SEARCH
     PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_lambda_first<BS, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, B.nbr,
                                           B.ncnt, B.lam1, dbg_lam, L, T.xcd_map)));
 }

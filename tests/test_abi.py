@@ -129,13 +129,13 @@ def test_create_without_gpu_is_an_error_not_a_crash():
     assert ei.value.code == pbf_amd.PBF_ERR_NOGPU
 
 
-def test_create_rejects_too_many_particles():
-    """Particle counts of 2^28 and more are refused before any device work: float4 slots are
-    addressed with 32-bit byte offsets."""
+@pytest.mark.parametrize("n", [(1 << 28) - 32, 1 << 28])
+def test_create_rejects_too_many_particles(n):
+    """Particle counts of 2^28 - 32 and more are refused before any device work: float4 slots,
+    the search's 32 padding slots included, are addressed with 32-bit byte offsets."""
     from pbf_amd import scenes
 
     sc = scenes.dam_break((4, 4, 4))
-    n = 1 << 28
     para = SolverParameter.from_dict(dict(sc.params), n, sc.grid_dims, 0)
     box = (C.c_float * 6)(*sc.box)
     bnd = (C.c_float * 6)(*sc.boundary)
