@@ -306,6 +306,7 @@ def main():
     ap.add_argument("--copy-mb", type=int, default=512, help="size of the calibration copy kernel (0: skip)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start leg")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--no-app", action="store_true", help="skip the reference app's scene-4 leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -427,7 +428,7 @@ def main():
         }
 
     app = None
-    if args.workload == "dam_break_262k":
+    if args.workload == "dam_break_262k" and not args.no_app:
         progress("the reference app's scene 4")
         app = reference_app_leg(args.warmup, args.steps)
 
