@@ -294,18 +294,21 @@ def test_neighbour_cap_cluster():
     compare_step(g, o)
 
 
-@pytest.mark.parametrize("cap", [12, 100, 255])
-def test_neighbour_cap_values(cap):
+@pytest.mark.parametrize("cap,block", [(12, 256), (100, 256), (255, 256), (255, 512), (216, 512), (217, 512)])
+def test_neighbour_cap_values(cap, block):
     """Caps other than the reference's 63: a list table of ceil(cap / 8) groups, truncation at
     the cap inside a dense cluster (about 200 particles within 0.04 m), and padding of the last
-    group.  The lists of the first step (cluster intact), then the state after a second step,
-    bit for bit."""
+    group.  The search's LDS rows leave room for hits past the cap when a workgroup's rows fit in
+    64 KiB (caps up to 216 at 512 lanes) and test every hit against the cap otherwise (217 and
+    255 at 512 lanes).  The lists of the first step (cluster intact), then the state after a
+    second step, bit for bit."""
     rng = np.random.default_rng(1)
     sc = scenes.dam_break((10, 10, 10), (0, 0, 0, 1.6, 1.6, 1.6), seed=11)
     pos = sc.pos.copy()
     pos[:200] = (np.array([0.8, 0.8, 0.8]) + rng.uniform(-0.02, 0.02, size=(200, 3))).astype(np.float32)
     sc.pos = pos
     g, o = pair(sc, max_neighbors=cap)
+    g.set_tuning(block, 0, True)
     g.step()
     o.step(1)
     on, oc = o.neighbors()
