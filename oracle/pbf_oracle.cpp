@@ -7,8 +7,11 @@
 // product (pbf_amd/) never links, loads or calls it.
 //
 // Arithmetic contract (shared with the HIP kernels, which must reproduce it bit for bit):
-//   * IEEE fp32 +,-,*,/ and sqrt only, evaluated left to right, no FMA contraction
-//     (built with -ffp-contract=off);
+//   * IEEE fp32 +,-,*,/ and sqrt, evaluated left to right (built with -ffp-contract=off, so the
+//     compiler fuses nothing), plus fused multiply-adds (std::fma, one rounding) written out at
+//     fixed places: a pair's r.r in the constraint, curl and vorticity passes (dot3f), the sums
+//     over neighbours there, and the Newton steps of pbf_rsqrt.  The neighbour search tests r.r
+//     left to right;
 //   * pow() of the reference is replaced by repeated multiplication for integer exponents;
 //   * the boundary jitter's sin() is the self-written pbf_sin below (Cody-Waite reduction +
 //     truncated Taylor series), not libm's, so it is identical on both sides;
@@ -74,6 +77,8 @@ struct V3 {
 
 inline V3 sub(const V3& a, const V3& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 inline float dot3(const V3& a, const V3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// r.r of a neighbour pair in the passes: x x, then two fused multiply-adds.
+inline float dot3f(const V3& r) { return std::fma(r.z, r.z, std::fma(r.y, r.y, r.x * r.x)); }
 inline V3 cross3(const V3& a, const V3& b) {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
@@ -145,8 +150,8 @@ inline float poly6(float r2, const Consts& K) {
 }
 
 // 1/sqrt(x): Lomont's refined bit-pattern seed 0x5f375a86 (C. Lomont, "Fast Inverse Square
-// Root", 2003) and three Newton steps y <- y (1.5 - ((x/2) y) y), plain IEEE fp32 (the kernels
-// use the same steps).
+// Root", 2003) and three Newton steps y <- y (1.5 - ((x/2) y) y), each 1.5 - ((x/2) y) y one fused
+// multiply-add (the kernels use the same steps).
 inline float pbf_rsqrt(float x) {
     uint32_t i;
     std::memcpy(&i, &x, sizeof(i));
@@ -154,9 +159,9 @@ inline float pbf_rsqrt(float x) {
     float y;
     std::memcpy(&y, &i, sizeof(y));
     const float hx = 0.5f * x;
-    y = y * (1.5f - (hx * y) * y);
-    y = y * (1.5f - (hx * y) * y);
-    y = y * (1.5f - (hx * y) * y);
+    y = y * std::fma(-(hx * y), y, 1.5f);
+    y = y * std::fma(-(hx * y), y, 1.5f);
+    y = y * std::fma(-(hx * y), y, 1.5f);
     return y;
 }
 
@@ -240,13 +245,13 @@ void orc_consts(const orc_params* P, float* out) {
 float orc_poly6(const orc_params* P, const float* r) {
     Consts K = make_consts(*P);
     V3 v{r[0], r[1], r[2]};
-    return poly6(dot3(v, v), K);
+    return poly6(dot3f(v), K);
 }
 
 void orc_spiky_grad(const orc_params* P, const float* r, float* g) {
     Consts K = make_consts(*P);
     V3 v{r[0], r[1], r[2]};
-    V3 o = spiky_grad(v, dot3(v, v), K);
+    V3 o = spiky_grad(v, dot3f(v), K);
     g[0] = o.x;
     g[1] = o.y;
     g[2] = o.z;
@@ -426,13 +431,13 @@ void orc_step(orc_state* s, int32_t nsteps) {
                 const int32_t* L = &s->nbr[(size_t)i * cap];
                 for (int j = 0; j < s->ncnt[i]; ++j) {
                     V3 r = sub(s->x[i], s->x[L[j]]);
-                    float r2 = dot3(r, r);
+                    float r2 = dot3f(r);
                     rho = rho + poly6(r2, K);
                     V3 g = spiky_grad(r, r2, K);
                     gradSum.x = gradSum.x + g.x;
                     gradSum.y = gradSum.y + g.y;
                     gradSum.z = gradSum.z + g.z;
-                    gradMagSum = gradMagSum + dot3(g, g);
+                    gradMagSum = std::fma(g.z, g.z, std::fma(g.y, g.y, std::fma(g.x, g.x, gradMagSum)));
                 }
                 float Ci = rho / K.restRho - 1.0f;
                 gradMagSum = gradMagSum + dot3(gradSum, gradSum);
@@ -449,13 +454,13 @@ void orc_step(orc_state* s, int32_t nsteps) {
                 for (int j = 0; j < s->ncnt[i]; ++j) {
                     int nid = L[j];
                     V3 r = sub(s->x[i], s->x[nid]);
-                    float r2 = dot3(r, r);
+                    float r2 = dot3f(r);
                     float w = poly6(r2, K);
                     V3 g = spiky_grad(r, r2, K);
                     float a = li + s->lam[nid] + scorr(w, K);
-                    acc.x = acc.x + a * g.x;
-                    acc.y = acc.y + a * g.y;
-                    acc.z = acc.z + a * g.z;
+                    acc.x = std::fma(a, g.x, acc.x);
+                    acc.y = std::fma(a, g.y, acc.y);
+                    acc.z = std::fma(a, g.z, acc.z);
                 }
                 s->dp[i] = {acc.x / K.restRho, acc.y / K.restRho, acc.z / K.restRho};
             }
@@ -481,11 +486,12 @@ void orc_step(orc_state* s, int32_t nsteps) {
                 int nid = L[j];
                 V3 vij = sub(s->v[nid], s->v[i]);
                 V3 r = sub(s->x[i], s->x[nid]);
-                float r2 = dot3(r, r);
-                V3 c = cross3(vij, spiky_grad(r, r2, K));
-                w.x = w.x + c.x;
-                w.y = w.y + c.y;
-                w.z = w.z + c.z;
+                float r2 = dot3f(r);
+                V3 g = spiky_grad(r, r2, K);
+                // w += vij x g, each component's two products fused into the sum
+                w.x = std::fma(vij.y, g.z, std::fma(-vij.z, g.y, w.x));
+                w.y = std::fma(vij.z, g.x, std::fma(-vij.x, g.z, w.y));
+                w.z = std::fma(vij.x, g.y, std::fma(-vij.y, g.x, w.z));
                 rho = rho + poly6(r2, K);
             }
             s->curl[i] = w;
@@ -500,16 +506,16 @@ void orc_step(orc_state* s, int32_t nsteps) {
                 int nid = L[j];
                 V3 vij = sub(s->v[i], s->v[nid]);
                 V3 r = sub(s->x[i], s->x[nid]);
-                float r2 = dot3(r, r);
+                float r2 = dot3f(r);
                 float w = poly6(r2, K);
-                pv.x = pv.x - vij.x * w;
-                pv.y = pv.y - vij.y * w;
-                pv.z = pv.z - vij.z * w;
+                pv.x = std::fma(-vij.x, w, pv.x);
+                pv.y = std::fma(-vij.y, w, pv.y);
+                pv.z = std::fma(-vij.z, w, pv.z);
                 float wl = std::sqrt(dot3(s->curl[nid], s->curl[nid]));
                 V3 g = spiky_grad(r, r2, K);
-                eta.x = eta.x + wl * g.x;
-                eta.y = eta.y + wl * g.y;
-                eta.z = eta.z + wl * g.z;
+                eta.x = std::fma(wl, g.x, eta.x);
+                eta.y = std::fma(wl, g.y, eta.y);
+                eta.z = std::fma(wl, g.z, eta.z);
             }
             float el = std::sqrt(dot3(eta, eta));
             if ((double)std::fabs(el) > 1e-3) eta = {eta.x / el, eta.y / el, eta.z / el};

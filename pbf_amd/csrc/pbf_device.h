@@ -1,15 +1,17 @@
 // Device-side math of one PBF step: smoothing kernels, boundary clamp with jitter and grid
 // hashing.  The formulas are those of the reference's kernels/cudaPBFSolver.cuh:99-181; the
 // arithmetic is fixed so that a CPU build of the same operations gives the same bits:
-//   * fp32 +,-,*,/ and sqrt only (correctly rounded; the library is built with
-//     -ffp-contract=off and -fhip-fp32-correctly-rounded-divide-sqrt), left to right;
+//   * fp32 +,-,*,/ and sqrt (correctly rounded; the library is built with -ffp-contract=off and
+//     -fhip-fp32-correctly-rounded-divide-sqrt), left to right, plus fused multiply-adds
+//     (correctly rounded, fmaf on the CPU) written out at fixed places: r.r and the sums over
+//     neighbours in the constraint, curl and vorticity passes, and the Newton steps of pbf_rsqrt;
 //   * 1/|r| of the spiky gradient from pbf_rsqrt (bit-pattern seed + Newton steps in those
 //     operations), not from sqrt and a division;
 //   * integer powers by repeated multiplication;
 //   * the jitter's sin is pbf_sin below, not the device libm's.
-// The fast mode (StepConsts::fast, PBF_FLAG_FAST_MATH) leaves that contract in the neighbour
-// terms of the constraint passes: fused multiply-adds and the hardware reciprocal square root
-// (the *_t<FAST> helpers below); its results are held to tolerances instead of bits.
+// The fast mode (StepConsts::fast, PBF_FLAG_FAST_MATH) leaves that contract in one place: the
+// spiky gradient's 1/|r| comes from the hardware reciprocal square root (spiky_scale_t<true>);
+// its results are held to tolerances instead of bits.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -110,14 +112,15 @@ __device__ __forceinline__ float poly6(float r2, const StepConsts& K) {
 }
 
 // 1/sqrt(x): Lomont's refined bit-pattern seed 0x5f375a86 (C. Lomont, "Fast Inverse Square
-// Root", 2003) and three Newton steps y <- y (1.5 - ((x/2) y) y), in IEEE fp32 with no
-// contraction, so the oracle computes the same bits.  Relative error ~2e-7 for normal x > 0.
+// Root", 2003) and three Newton steps y <- y (1.5 - ((x/2) y) y), each 1.5 - ((x/2) y) y one
+// fused multiply-add, in IEEE fp32, so the oracle computes the same bits.  Relative error below
+// 2e-7 for normal x > 0.
 __device__ __forceinline__ float pbf_rsqrt(float x) {
     float y = __uint_as_float(0x5f375a86u - (__float_as_uint(x) >> 1));
     const float hx = 0.5f * x;
-    y = y * (1.5f - (hx * y) * y);
-    y = y * (1.5f - (hx * y) * y);
-    y = y * (1.5f - (hx * y) * y);
+    y = y * __builtin_fmaf(-(hx * y), y, 1.5f);
+    y = y * __builtin_fmaf(-(hx * y), y, 1.5f);
+    y = y * __builtin_fmaf(-(hx * y), y, 1.5f);
     return y;
 }
 
@@ -133,18 +136,13 @@ __device__ __forceinline__ float spiky_scale(float r2, const StepConsts& K) {
     return len < K.rmin ? 0.0f : t;
 }
 
-// a * b + c: two IEEE operations (the bit-exact contract), or one fused multiply-add (FAST).
-template <bool FAST>
-__device__ __forceinline__ float madd(float a, float b, float c) {
-    if constexpr (FAST) return __builtin_fmaf(a, b, c);
-    return a * b + c;
-}
+// a * b + c as one fused multiply-add: a neighbour term into its sum.
+__device__ __forceinline__ float madd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
-// r.r: left to right (bit-exact contract) or with fused multiply-adds (FAST).
-template <bool FAST>
-__device__ __forceinline__ float dot3_t(float x, float y, float z) {
-    if constexpr (FAST) return __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x));
-    return x * x + y * y + z * z;
+// r.r of a neighbour pair as x x, then two fused multiply-adds.  (The neighbour search tests its
+// candidates with r.r left to right, as the reference's buildNeighborhood.)
+__device__ __forceinline__ float dot3f(float x, float y, float z) {
+    return __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x));
 }
 
 // spiky_scale with the hardware reciprocal square root (v_rsq_f32, about 1 ulp) in FAST mode.

@@ -361,11 +361,7 @@ struct LambdaAcc {
         gx = gx + ax;
         gy = gy + ay;
         gz = gz + az;
-        if constexpr (FAST) {
-            gms = __builtin_fmaf(az, az, __builtin_fmaf(ay, ay, __builtin_fmaf(ax, ax, gms)));
-        } else {
-            gms = gms + (ax * ax + ay * ay + az * az);
-        }
+        gms = __builtin_fmaf(az, az, __builtin_fmaf(ay, ay, __builtin_fmaf(ax, ax, gms)));
     }
     __device__ __forceinline__ float lambda(const StepConsts& K) const {
         float Ci = rho / K.restRho - 1.0f;
@@ -620,7 +616,7 @@ __global__ __launch_bounds__(BS) void k_lambda_first(StepConsts K, float4* __res
         A.init();
         for_neighbors(nbr, s, K.nstride, cnt, pos, [&](uint32_t, const float4& xj) {
             float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-            A.template add<FAST>(rx, ry, rz, dot3_t<FAST>(rx, ry, rz), K);
+            A.template add<FAST>(rx, ry, rz, dot3f(rx, ry, rz), K);
         });
         const float lam = A.lambda(K);
         reinterpret_cast<float*>(pos + s)[3] = lam;
@@ -653,7 +649,7 @@ __global__ __launch_bounds__(BS) void k_lambda(StepConsts K, const SceneGeom* __
         A.init();
         for_neighbors(nbr, s, K.nstride, cnt, pos, [&](uint32_t, const float4& xj) {
             float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-            A.template add<FAST>(rx, ry, rz, dot3_t<FAST>(rx, ry, rz), K);
+            A.template add<FAST>(rx, ry, rz, dot3f(rx, ry, rz), K);
         });
         const float lam = A.lambda(K);
         reinterpret_cast<float*>(pos + s)[3] = lam;
@@ -721,14 +717,14 @@ __global__ __launch_bounds__(BS) void k_delta_apply(StepConsts K, const SceneGeo
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
         for_neighbors(nbr, s, K.nstride, cnt, pos, [&](uint32_t q, const float4& xj) {
             float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-            float r2 = dot3_t<FAST>(rx, ry, rz);
+            float r2 = dot3f(rx, ry, rz);
             float w = poly6(r2, K);
             float t = spiky_scale_t<FAST>(r2, K);
             const float lam_j = LS ? LA.lam_src[q >> 4] : xj.w;
             float a = (lam_i + lam_j) + (SN == 4 ? scorr4(w, K) : scorr(w, K));
-            ax = madd<FAST>(a, rx * t, ax);
-            ay = madd<FAST>(a, ry * t, ay);
-            az = madd<FAST>(a, rz * t, az);
+            ax = madd(a, rx * t, ax);
+            ay = madd(a, ry * t, ay);
+            az = madd(a, rz * t, az);
         });
         float x = xi.x + ax / K.restRho;
         float y = xi.y + ay / K.restRho;
@@ -780,22 +776,16 @@ __global__ __launch_bounds__(BS) void k_curl(StepConsts K, float4* __restrict__ 
                 const float4 xj = xs[u], vj = vs[u];
                 float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
                 float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-                float r2 = dot3_t<FAST>(rx, ry, rz);
+                float r2 = dot3f(rx, ry, rz);
                 float t = spiky_scale_t<FAST>(r2, K);
                 float gx = rx * t, gy = ry * t, gz = rz * t;
-                if constexpr (FAST) {
-                    wx = __builtin_fmaf(uy, gz, __builtin_fmaf(-uz, gy, wx));
-                    wy = __builtin_fmaf(uz, gx, __builtin_fmaf(-ux, gz, wy));
-                    wz = __builtin_fmaf(ux, gy, __builtin_fmaf(-uy, gx, wz));
-                } else {
-                    wx = wx + (uy * gz - uz * gy);
-                    wy = wy + (uz * gx - ux * gz);
-                    wz = wz + (ux * gy - uy * gx);
-                }
+                wx = __builtin_fmaf(uy, gz, __builtin_fmaf(-uz, gy, wx));
+                wy = __builtin_fmaf(uz, gx, __builtin_fmaf(-ux, gz, wy));
+                wz = __builtin_fmaf(ux, gy, __builtin_fmaf(-uy, gx, wz));
                 const float w = poly6(r2, K);
-                px = madd<FAST>(ux, w, px);
-                py = madd<FAST>(uy, w, py);
-                pz = madd<FAST>(uz, w, pz);
+                px = madd(ux, w, px);
+                py = madd(uy, w, py);
+                pz = madd(uz, w, pz);
                 if constexpr (STATS) rho = rho + w;
             }
         }
@@ -861,11 +851,11 @@ __global__ __launch_bounds__(BS) void k_vort_apply(StepConsts K, const SceneGeom
         for_neighbors(nbr, s, K.nstride, cnt, pos, [&](uint32_t, const float4& xj) {
             const float wl = xj.w;  // |omega_j|
             float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-            float r2 = dot3_t<FAST>(rx, ry, rz);
+            float r2 = dot3f(rx, ry, rz);
             float t = spiky_scale_t<FAST>(r2, K);
-            ex = madd<FAST>(wl, rx * t, ex);
-            ey = madd<FAST>(wl, ry * t, ey);
-            ez = madd<FAST>(wl, rz * t, ez);
+            ex = madd(wl, rx * t, ex);
+            ey = madd(wl, ry * t, ey);
+            ez = madd(wl, rz * t, ez);
         });
         float el = sqrtf(ex * ex + ey * ey + ez * ez);
         if ((double)fabsf(el) > 1e-3) {
@@ -1069,7 +1059,7 @@ void launch_sort_reorder(hipStream_t st, const StepConsts& K, const Geometry& GE
                        B.cell, B.cstart, vel_in, pos_in, pos_out, vel_out, B.xold, ids_out, dbg_gid);
 }
 
-// The FAST instances (fused multiply-adds, hardware rsqrt) when the step constants ask for them.
+// The FAST instances (hardware rsqrt in the spiky gradient) when the step constants ask for them.
 #define PBF_DISPATCH_FAST(K, CALL)                 \
     if ((K).fast) {                                \
         constexpr bool FAST = true;                \

@@ -20,7 +20,7 @@ MLP = "tests/test_mlp_oracle.py::"
 
 # (name, file, original text, planted text, pin tests that must fail)
 MUTANTS = [
-    ("xsph_sign", "pbf_oracle.cpp", "pv.x = pv.x - vij.x * w;", "pv.x = pv.x + vij.x * w;",
+    ("xsph_sign", "pbf_oracle.cpp", "pv.x = std::fma(-vij.x, w, pv.x);", "pv.x = std::fma(vij.x, w, pv.x);",
      [VEL + "test_curl_and_xsph_against_reference_cpu_solver", VEL + "test_xsph_and_apply_l_shape_closed_form"]),
     ("xsph_own_velocity", "pbf_oracle.cpp", "V3 vij = sub(s->v[i], s->v[nid]);", "V3 vij = s->v[i];",
      [VEL + "test_xsph_conserves_momentum_on_a_lattice", VEL + "test_curl_and_xsph_against_reference_cpu_solver"]),
