@@ -390,7 +390,7 @@ def main():
     finite = bool(np.isfinite(pos).all() and np.isfinite(vel).all())
     s.close()
 
-    # the fast-arithmetic mode (fused multiply-adds, hardware rsqrt in the neighbour terms): same
+    # the fast-arithmetic mode (hardware rsqrt in the neighbour terms): same
     # workload, warm-up and launch shape
     progress("fast-arithmetic leg")
     if batch is not None:

@@ -37,9 +37,9 @@ extern "C" {
 #define PBF_FLAG_DEBUG 1u         /* keep per-step intermediates for pbf_get_grid / _neighbors / _fields */
 #define PBF_FLAG_DENSITY_STATS 2u /* compute density-error statistics every step */
 #define PBF_FLAG_GRAPH 4u         /* replay each step from a captured hipGraph */
-#define PBF_FLAG_FAST_MATH 8u     /* fast arithmetic in the constraint passes' neighbour terms: fused
-                                     multiply-adds and the hardware reciprocal square root.  Results
-                                     then match the bit-exact default to tolerances, not bits. */
+#define PBF_FLAG_FAST_MATH 8u     /* fast arithmetic in the constraint passes' neighbour terms: the
+                                     hardware reciprocal square root for 1/|r|.  Results then match
+                                     the bit-exact default to tolerances, not bits. */
 
 /* Solver parameters.  The first 13 fields are the reference's SolverParameter
  * (include/solverBase.h:16-40) in the same order, size and meaning.  restRho is the reference's

@@ -1,7 +1,8 @@
 """GPU parity: the HIP solver (through the C ABI) against the oracle on identical inputs.
 
 The kernels and the oracle follow one arithmetic contract (IEEE fp32 +,-,*,/,sqrt, left to
-right, no contraction, a shared self-written sin for the clamp jitter), so everything is
+right, fused multiply-adds only where both write them out, a shared self-written sin for the clamp
+jitter), so everything is
 compared bit for bit: cell ids, counts, cell starts, sorted order, neighbour lists, lambda,
 curl, density error and the state after one and after many steps.
 """
