@@ -123,8 +123,9 @@ def test_bench_reads_the_committed_counter_summaries():
     for stage in ("vort_apply_hash", "lambda_fast", "delta_apply_fast", "curl_fast", "vort_apply_hash_fast"):
         assert bench.measured_traffic("dam_break_262k", stage) > 1e6
         assert bench.measured_valu("dam_break_262k", stage) > 1e6
-    # the fast mode issues fewer VALU instructions per launch
+    # the fast mode issues fewer VALU instructions per launch (the hardware rsqrt in place of three
+    # Newton steps, about a fifth of a neighbour term)
     for stage in ("delta_apply", "lambda"):
-        assert bench.measured_valu("dam_break_262k", stage + "_fast") < 0.75 * bench.measured_valu("dam_break_262k", stage)
+        assert bench.measured_valu("dam_break_262k", stage + "_fast") < 0.85 * bench.measured_valu("dam_break_262k", stage)
     assert bench.measured_valu("dam_break_262k", "no_such_stage") is None
     assert bench.measured_valu("no_such_workload", "delta_apply") is None
