@@ -404,20 +404,60 @@ __device__ __forceinline__ void load_group(const uint32_t* __restrict__ nbr, int
     q[7] = b.w;
 }
 
-// Calls f(qb, x_q) for the list entries j = 0 .. ceil8(cnt) - 1 of slot s in order, qb = 16 q the
-// entry's byte offset and x_q = pos[q] (entries past cnt are the padding slot).
+// The largest list length among the wavefront's active lanes, as a wave-uniform value (counts are
+// below 256; one ballot per bit).  Past it every active lane's entries are the padding slot, so
+// the neighbour loops end there instead of at the end of the group: VALU instructions issue for
+// the whole wavefront, and the padding terms (+-0 into sums that start at +0) change nothing.
+__device__ __forceinline__ int wave_max_count(int cnt) {
+    int m = 0;
+#pragma unroll
+    for (int b = 7; b >= 0; --b) {
+        const int t = m | (1 << b);
+        if (__ballot(cnt >= t)) m = t;
+    }
+    return m;
+}
 
+// g(0) .. g(k - 1) for one group, k = rem (the entries left before the wavefront's largest count)
+// rounded up to even and at most 8.  The tests before entries 2, 4 and 6 are wave-uniform
+// branches; a branch per entry, or a separate branch-free copy for whole groups, measured slower.
+template <class G>
+__device__ __forceinline__ void run_group(int rem, G&& g) {
+    static_assert(kGroup == 8, "the tail below is written out for groups of 8");
+    g(0);
+    g(1);
+    if (rem > 2) {
+        g(2);
+        g(3);
+        if (rem > 4) {
+            g(4);
+            g(5);
+            if (rem > 6) {
+                g(6);
+                g(7);
+            }
+        }
+    }
+}
+
+// Calls f(qb, x_q) for the list entries j = 0 .. min(ceil8(cnt), ceil2(cmax)) - 1 of slot s in
+// order, qb = 16 q the entry's byte offset and x_q = pos[q] (entries past cnt are the padding
+// slot), cmax the wavefront's largest count.
 template <class F>
 __device__ __forceinline__ void for_neighbors(const uint32_t* __restrict__ nbr, int s, int nstride, int cnt,
                                               const float4* __restrict__ pos, F&& f) {
+    const int cmax = wave_max_count(cnt);
     for (int j0 = 0; j0 < cnt; j0 += kGroup) {
         uint32_t q[kGroup];
         load_group(nbr, s, j0, nstride, q);
         float4 x[kGroup];
 #pragma unroll
         for (int u = 0; u < kGroup; ++u) x[u] = slot_at(pos, q[u]);
-#pragma unroll
-        for (int u = 0; u < kGroup; ++u) f(q[u], x[u]);
+        // The gathers are used here so that the compiler issues all of them before the tail
+        // branches below instead of sinking each into its branch behind a wait of its own.
+        asm volatile("" ::"v"(x[0].x), "v"(x[1].x), "v"(x[2].x), "v"(x[3].x), "v"(x[4].x), "v"(x[5].x),
+                     "v"(x[6].x), "v"(x[7].x));
+        run_group(cmax - j0, [&](int u) { f(q[u], x[u]); });
     }
 }
 
@@ -762,6 +802,7 @@ __global__ __launch_bounds__(BS) void k_curl(StepConsts K, float4* __restrict__ 
         const float4 vi = vel[s];
         int cnt = ncnt[s];
         float wx = 0.0f, wy = 0.0f, wz = 0.0f, rho = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
+        const int cmax = wave_max_count(cnt);
         for (int j0 = 0; j0 < cnt; j0 += kGroup) {
             uint32_t qs[kGroup];
             float4 xs[kGroup], vs[kGroup];
@@ -771,8 +812,12 @@ __global__ __launch_bounds__(BS) void k_curl(StepConsts K, float4* __restrict__ 
                 xs[u] = slot_at(pos, qs[u]);
                 vs[u] = slot_at(vel, qs[u]);
             }
-#pragma unroll
-            for (int u = 0; u < kGroup; ++u) {
+            // all gathers issued ahead of the tail branches (as in for_neighbors)
+            asm volatile("" ::"v"(xs[0].x), "v"(xs[1].x), "v"(xs[2].x), "v"(xs[3].x), "v"(xs[4].x),
+                         "v"(xs[5].x), "v"(xs[6].x), "v"(xs[7].x));
+            asm volatile("" ::"v"(vs[0].x), "v"(vs[1].x), "v"(vs[2].x), "v"(vs[3].x), "v"(vs[4].x),
+                         "v"(vs[5].x), "v"(vs[6].x), "v"(vs[7].x));
+            run_group(cmax - j0, [&](int u) {
                 const float4 xj = xs[u], vj = vs[u];
                 float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
                 float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
@@ -787,7 +832,7 @@ __global__ __launch_bounds__(BS) void k_curl(StepConsts K, float4* __restrict__ 
                 py = madd(uy, w, py);
                 pz = madd(uz, w, pz);
                 if constexpr (STATS) rho = rho + w;
-            }
+            });
         }
         const float wlen = sqrtf(wx * wx + wy * wy + wz * wz);
         omega[s] = make_float4(wx, wy, wz, wlen);
