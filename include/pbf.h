@@ -107,9 +107,9 @@ int pbf_set_ordering(pbf_solver* s, int32_t order);
 int pbf_get_ordering(pbf_solver* s, int32_t* out);
 
 /* Autotuner: times every launch shape over `steps` steps (rounded up to whole multi-step graphs)
- * replayed as pbf_advance replays them, keeps the fastest, then times that shape with the other
- * row order (pbf_set_ordering) and keeps the faster; it restores the state (tuning does not
- * advance the simulation).  The last step's diagnostics (pbf_get_grid, _neighbors, _fields,
+ * replayed as pbf_advance replays them, each from the state it was called on, keeps the fastest,
+ * then times that shape with the other row order (pbf_set_ordering) and keeps the faster; it
+ * restores the state (tuning does not advance the simulation).  The last step's diagnostics (pbf_get_grid, _neighbors, _fields,
  * pbf_density_stats) are invalid until the next step.  best[3] = block, grid, xcd_map; best_ms =
  * time per step; the order chosen is pbf_get_ordering's.  Either may be NULL.  No counterpart in
  * the reference. */

@@ -1125,11 +1125,28 @@ int pbf_autotune(pbf_solver* s, int32_t steps, int32_t* best, float* best_ms) {
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    // ms per step of the current shape and order, stepped as pbf_advance / pbf_time_steps step:
-    // `steps` rounded up to whole multi-step graphs, captured first, after one warm-up graph
+    // puts the solver back on the state the tuning started from (every candidate is timed from it:
+    // the work per step depends on the state, and a candidate timed on later steps would be
+    // charged for them)
+    auto restore = [&]() -> int {
+        s->pc = pc0;
+        s->vc = vc0;
+        s->ic = ic0;
+        s->steps_done = sd0;
+        drop_graphs(s);
+        invalidate_hash(s);  // the tuning steps hashed their own successors
+        hipError_t e = hipMemcpyAsync(s->B.pos[s->pc], sp, sizeof(float4) * np, hipMemcpyDeviceToDevice, s->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(s->B.vel[s->vc], sv, sizeof(float4) * np, hipMemcpyDeviceToDevice, s->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(s->B.ids[s->ic], si, sizeof(uint32_t) * np, hipMemcpyDeviceToDevice, s->stream);
+        if (e != hipSuccess) return set_error(PBF_ERR_HIP, "state restore failed: %s", hipGetErrorString(e));
+        return PBF_OK;
+    };
+    // ms per step of the current shape and order from the starting state, stepped as pbf_advance /
+    // pbf_time_steps step: `steps` rounded up to whole multi-step graphs, captured first, after one
+    // warm-up graph
     const int rounds = ((steps + kGraphSteps - 1) / kGraphSteps) * kGraphSteps;
     auto time_config = [&](float& ms) -> int {
-        drop_graphs(s);
+        if (int r = restore()) return r;
         if (int r = prepare_graphs(s, rounds)) return r;
         if (int r = run_many(s, kGraphSteps)) return r;
         if (hipEventRecord(e0, s->stream) != hipSuccess) return set_error(PBF_ERR_HIP, "event");
@@ -1170,25 +1187,18 @@ int pbf_autotune(pbf_solver* s, int32_t steps, int32_t* best, float* best_ms) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     // restore the state the tuning started from
-    s->pc = pc0;
-    s->vc = vc0;
-    s->ic = ic0;
-    s->steps_done = sd0;
     s->P.flags = flags0;
     // the tuning steps overwrote the spare buffers, lambda and the cell tables: the last step's
     // diagnostics (lambda, density statistics, grid, lists) are gone until the next step
     s->lam_last = -1;
     s->stats_valid = false;
     s->diag_valid = false;
-    drop_graphs(s);
-    invalidate_hash(s);  // the tuning steps hashed their own successors
-    hipError_t e = hipMemcpyAsync(s->B.pos[s->pc], sp, sizeof(float4) * np, hipMemcpyDeviceToDevice, s->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(s->B.vel[s->vc], sv, sizeof(float4) * np, hipMemcpyDeviceToDevice, s->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(s->B.ids[s->ic], si, sizeof(uint32_t) * np, hipMemcpyDeviceToDevice, s->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    const int rr = restore();
+    hipError_t e = hipStreamSynchronize(s->stream);
     (void)hipFree(sp);
     (void)hipFree(sv);
     (void)hipFree(si);
+    if (rr) return rr;
     if (e != hipSuccess) return set_error(PBF_ERR_HIP, "state restore failed: %s", hipGetErrorString(e));
     s->tune = rc == PBF_OK ? bestT : keep;
     if (rc) return rc;
