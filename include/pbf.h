@@ -222,7 +222,7 @@ int pbf_stage_times(pbf_solver* s, float out[7]);
 int pbf_profile_step(pbf_solver* s, float* ms, const char** names, int32_t max);
 
 /* ---- learned warm start and adaptive iteration (no counterpart in the reference) ----------
- * A per-particle MLP 16 -> 64 -> 64 -> 64 -> 1 (SiLU), BF16 weights and activations, FP32
+ * A per-particle MLP 16 -> 64 -> 64 -> 64 -> 1 (hard-swish), BF16 weights and activations, FP32
  * accumulation on MFMA, predicts the multiplier a plain step accumulates over its Jacobi
  * iterations 2..I (divided by h^2) from 16 features of the particle after the plain pass 1:
  *   0 C = rho/restRho - 1   1 neighbour count   2 sum_j |grad_j C|^2 / restRho^2

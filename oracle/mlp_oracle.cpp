@@ -3,10 +3,13 @@
 // The learned warm start has no counterpart in the reference; this file states the network the
 // HIP kernels implement so their forward pass, gradients and optimiser step can be checked:
 //
-//   x (16) -> z1 = W1 x + b1 (64) -> h1 = bf16(silu(z1))
-//          -> z2 = W2 h1 + b2 (64) -> h2 = bf16(silu(z2))
-//          -> z3 = W3 h2 + b3 (64) -> h3 = bf16(silu(z3))
+//   x (16) -> z1 = W1 x + b1 (64) -> h1 = bf16(act(z1))
+//          -> z2 = W2 h1 + b2 (64) -> h2 = bf16(act(z2))
+//          -> z3 = W3 h2 + b3 (64) -> h3 = bf16(act(z3))
 //          -> y  = W4 h3 + b4 (1)
+//
+// act is hard-swish, z * min(max(z + 3, 0), 6) / 6, with torch's derivative at the kinks (0 below
+// -3, z / 3 + 1/2 from -3 up to and including 3, 1 above).
 //
 // Weights W are row-major [out][in].  Inputs x and weights are BF16 on the device; products are
 // accumulated in FP32 there and in FP64 here.  bf16() is round-to-nearest-even of the FP32
@@ -14,6 +17,7 @@
 // skipped, which gives the smooth FP64 network used for the finite-difference check of the
 // analytic backward.  Loss: mean squared error over the batch.
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -34,11 +38,8 @@ inline float bf16_round(float f) {
     return r;
 }
 
-inline double silu(double z) { return z / (1.0 + std::exp(-z)); }
-inline double dsilu(double z) {
-    double s = 1.0 / (1.0 + std::exp(-z));
-    return s * (1.0 + z * (1.0 - s));
-}
+inline double act(double z) { return z * std::min(std::max(z + 3.0, 0.0), 6.0) / 6.0; }
+inline double dact(double z) { return z < -3.0 ? 0.0 : (z <= 3.0 ? z / 3.0 + 0.5 : 1.0); }
 
 struct Params {
     const float *W1, *b1, *W2, *b2, *W3, *b3, *W4, *b4;
@@ -76,19 +77,19 @@ void mlp_forward(const float* params, const float* x, int nb, int bf16_mode, dou
         for (int o = 0; o < H; ++o) {
             double z = P.b1[o];
             for (int i = 0; i < IN; ++i) z += w(P.W1[o * IN + i]) * a0[i];
-            double h = silu(z);
+            double h = act(z);
             a1[o] = bf16_mode ? (double)bf16_round((float)h) : h;
         }
         for (int o = 0; o < H; ++o) {
             double z = P.b2[o];
             for (int i = 0; i < H; ++i) z += w(P.W2[o * H + i]) * a1[i];
-            double h = silu(z);
+            double h = act(z);
             a2[o] = bf16_mode ? (double)bf16_round((float)h) : h;
         }
         for (int o = 0; o < H; ++o) {
             double z = P.b3[o];
             for (int i = 0; i < H; ++i) z += w(P.W3[o * H + i]) * a2[i];
-            double h = silu(z);
+            double h = act(z);
             a3[o] = bf16_mode ? (double)bf16_round((float)h) : h;
         }
         double yy = P.b4[0];
@@ -121,21 +122,21 @@ double mlp_loss_grad(const float* params, const float* x, const float* t, int nb
             double z = P.b1[o];
             for (int i = 0; i < IN; ++i) z += w(P.W1[o * IN + i]) * a0[i];
             z1[o] = z;
-            double h = silu(z);
+            double h = act(z);
             a1[o] = bf16_mode ? (double)bf16_round((float)h) : h;
         }
         for (int o = 0; o < H; ++o) {
             double z = P.b2[o];
             for (int i = 0; i < H; ++i) z += w(P.W2[o * H + i]) * a1[i];
             z2[o] = z;
-            double h = silu(z);
+            double h = act(z);
             a2[o] = bf16_mode ? (double)bf16_round((float)h) : h;
         }
         for (int o = 0; o < H; ++o) {
             double z = P.b3[o];
             for (int i = 0; i < H; ++i) z += w(P.W3[o * H + i]) * a2[i];
             z3[o] = z;
-            double h = silu(z);
+            double h = act(z);
             a3[o] = bf16_mode ? (double)bf16_round((float)h) : h;
         }
         double y = P.b4[0];
@@ -146,7 +147,7 @@ double mlp_loss_grad(const float* params, const float* x, const float* t, int nb
         gb4[0] += dy;
         for (int i = 0; i < H; ++i) {
             gW4[i] += dy * a3[i];
-            d3[i] = dy * w(P.W4[i]) * dsilu(z3[i]);
+            d3[i] = dy * w(P.W4[i]) * dact(z3[i]);
         }
         for (int o = 0; o < H; ++o) {
             gb3[o] += d3[o];
@@ -155,7 +156,7 @@ double mlp_loss_grad(const float* params, const float* x, const float* t, int nb
         for (int i = 0; i < H; ++i) {
             double s = 0.0;
             for (int o = 0; o < H; ++o) s += w(P.W3[o * H + i]) * d3[o];
-            d2[i] = s * dsilu(z2[i]);
+            d2[i] = s * dact(z2[i]);
         }
         for (int o = 0; o < H; ++o) {
             gb2[o] += d2[o];
@@ -164,7 +165,7 @@ double mlp_loss_grad(const float* params, const float* x, const float* t, int nb
         for (int i = 0; i < H; ++i) {
             double s = 0.0;
             for (int o = 0; o < H; ++o) s += w(P.W2[o * H + i]) * d2[o];
-            d1[i] = s * dsilu(z1[i]);
+            d1[i] = s * dact(z1[i]);
         }
         for (int o = 0; o < H; ++o) {
             gb1[o] += d1[o];

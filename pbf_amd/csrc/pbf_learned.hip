@@ -1,4 +1,4 @@
-// Learned warm start: a per-particle MLP 16 -> 64 -> 64 -> 64 -> 1 (SiLU) that predicts the
+// Learned warm start: a per-particle MLP 16 -> 64 -> 64 -> 64 -> 1 (hard-swish) that predicts the
 // constraint multiplier a plain solve accumulates over its Jacobi iterations, so the first
 // delta-p pass can start from it.  No counterpart in the reference.
 //
@@ -33,14 +33,15 @@ constexpr int kB = 256;
 constexpr int LD64 = 72;  // LDS row pitch (bf16) of 64-wide rows: 144 B spreads the banks
 constexpr int LD16 = 24;  // ... of 16-wide rows
 
-// The network computes in BF16 (activations are rounded to 8 significant bits right after), so
-// the sigmoid takes the hardware exp and reciprocal (about 1 ulp) rather than the
-// correctly-rounded division this library is otherwise built with.
-__device__ __forceinline__ float sigmoidf(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
-__device__ __forceinline__ float siluf(float z) { return z * sigmoidf(z); }
-__device__ __forceinline__ float dsiluf(float z) {
-    float s = sigmoidf(z);
-    return s * (1.0f + z * (1.0f - s));
+// Hard-swish, h = z * clamp(z + 3, 0, 6) / 6 (A. Howard et al., "Searching for MobileNetV3",
+// 2019): SiLU's shape from full-rate operations (an add, a clamp, two multiplies).  SiLU's exp and
+// reciprocal issue at a quarter of the rate and took about half of the inference kernel's issue
+// cycles.  The derivative is torch's: 0 below -3, z / 3 + 1/2 up to 3, 1 above.
+__device__ __forceinline__ float actf(float z) {
+    return (z * __builtin_amdgcn_fmed3f(z + 3.0f, 0.0f, 6.0f)) * (1.0f / 6.0f);
+}
+__device__ __forceinline__ float dactf(float z) {
+    return z < -3.0f ? 0.0f : (z <= 3.0f ? z * (1.0f / 3.0f) + 0.5f : 1.0f);
 }
 
 // acc += A[32 x K] * B[32 x K]^T, rows of A at a + r*lda, rows of B at b + r*ldb (bf16,
@@ -65,6 +66,14 @@ __device__ __forceinline__ f32x16 zero16() {
     return z;
 }
 
+// The accumulator of feature tile ft preset to the layer's biases, so that the MFMAs produce
+// W x + b (no add per element afterwards).
+__device__ __forceinline__ f32x16 bias16(const float* __restrict__ bias, int ft) {
+    f32x16 z;
+    for (int e = 0; e < 16; ++e) z[e] = bias[ft * 32 + acc_row(e)];
+    return z;
+}
+
 // Standardised, clipped, bf16 features of 64 samples into Xs[64][LD16] and XsT[32][LD64]
 // (rows 16..31 of XsT are zero).  src(n) gives the raw feature row of sample n or NULL.
 template <class Src>
@@ -84,15 +93,14 @@ __device__ void load_features(const MlpModel* __restrict__ M, Src src, bf16* Xs,
 }
 
 // One hidden layer for this wave's tile: z = W[ft rows] * in^T[sh cols] + b.  Writes
-// bf16(silu(z)) sample-major into out[64][LD64] and, if outT, feature-major into outT[64][LD64].
+// bf16(act(z)) sample-major into out[64][LD64] and, if outT, feature-major into outT[64][LD64].
 __device__ __forceinline__ f32x16 hidden_layer(const bf16* W, int ldw, int K, const float* bias, const bf16* in, int ldin,
                                                bf16* out, bf16* outT, int ft, int sh) {
-    f32x16 z = mma_nt(W + ft * 32 * ldw, ldw, in + sh * 32 * ldin, ldin, K, zero16());
+    f32x16 z = mma_nt(W + ft * 32 * ldw, ldw, in + sh * 32 * ldin, ldin, K, bias16(bias, ft));
     const int n = sh * 32 + acc_col();
     for (int e = 0; e < 16; ++e) {
         int o = ft * 32 + acc_row(e);
-        z[e] = z[e] + bias[o];
-        bf16 h = (bf16)siluf(z[e]);
+        bf16 h = (bf16)actf(z[e]);
         out[n * LD64 + o] = h;
         if (outT) outT[o * LD64 + n] = h;
     }
@@ -116,11 +124,11 @@ __global__ __launch_bounds__(kB) void k_mlp_infer(const MlpModel* __restrict__ M
         __syncthreads();
         hidden_layer(M->W2, 64, 64, M->b2, H1, LD64, H2, nullptr, ft, sh);
         __syncthreads();
-        f32x16 z3 = mma_nt(M->W3 + ft * 32 * 64, 64, H2 + sh * 32 * LD64, LD64, 64, zero16());
+        f32x16 z3 = mma_nt(M->W3 + ft * 32 * 64, 64, H2 + sh * 32 * LD64, LD64, 64, bias16(M->b3, ft));
         float part = 0.0f;
         for (int e = 0; e < 16; ++e) {
             int o = ft * 32 + acc_row(e);
-            float h = (float)(bf16)siluf(z3[e] + M->b3[o]);
+            float h = (float)(bf16)actf(z3[e]);
             part = part + M->w4[o] * h;
         }
         part = part + __shfl_xor(part, 32);
@@ -171,13 +179,12 @@ __global__ __launch_bounds__(kB) void k_mlp_grad(const MlpModel* __restrict__ M,
         __syncthreads();
         f32x16 z2 = hidden_layer(M->W2, 64, 64, M->b2, H1, LD64, H2, H2T, ft, sh);
         __syncthreads();
-        f32x16 z3 = mma_nt(M->W3 + ft * 32 * 64, 64, H2 + sh * 32 * LD64, LD64, 64, zero16());
+        f32x16 z3 = mma_nt(M->W3 + ft * 32 * 64, 64, H2 + sh * 32 * LD64, LD64, 64, bias16(M->b3, ft));
         float part = 0.0f;
         const int ncol = sh * 32 + acc_col();
         for (int e = 0; e < 16; ++e) {
             int o = ft * 32 + acc_row(e);
-            z3[e] = z3[e] + M->b3[o];
-            bf16 hb = (bf16)siluf(z3[e]);
+            bf16 hb = (bf16)actf(z3[e]);
             D[ncol * LD64 + o] = hb;  // h3, sample-major, for the W4 gradient
             part = part + M->w4[o] * (float)hb;
         }
@@ -202,10 +209,10 @@ __global__ __launch_bounds__(kB) void k_mlp_grad(const MlpModel* __restrict__ M,
             gw4 = gw4 + s;
         }
         __syncthreads();
-        // dz3 = dy * w4 * silu'(z3)
+        // dz3 = dy * w4 * act'(z3)
         for (int e = 0; e < 16; ++e) {
             int o = ft * 32 + acc_row(e);
-            bf16 g = (bf16)(dy[ncol] * M->w4[o] * dsiluf(z3[e]));
+            bf16 g = (bf16)(dy[ncol] * M->w4[o] * dactf(z3[e]));
             D[ncol * LD64 + o] = g;
             DT[o * LD64 + ncol] = g;
         }
@@ -220,7 +227,7 @@ __global__ __launch_bounds__(kB) void k_mlp_grad(const MlpModel* __restrict__ M,
         __syncthreads();
         for (int e = 0; e < 16; ++e) {
             int o = ft * 32 + acc_row(e);
-            bf16 g = (bf16)(d2[e] * dsiluf(z2[e]));
+            bf16 g = (bf16)(d2[e] * dactf(z2[e]));
             D[ncol * LD64 + o] = g;
             DT[o * LD64 + ncol] = g;
         }
@@ -235,7 +242,7 @@ __global__ __launch_bounds__(kB) void k_mlp_grad(const MlpModel* __restrict__ M,
         __syncthreads();
         for (int e = 0; e < 16; ++e) {
             int o = ft * 32 + acc_row(e);
-            DT[o * LD64 + ncol] = (bf16)(d1[e] * dsiluf(z1[e]));
+            DT[o * LD64 + ncol] = (bf16)(d1[e] * dactf(z1[e]));
         }
         __syncthreads();
         if (tid < 64) {

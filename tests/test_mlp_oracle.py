@@ -1,9 +1,8 @@
 """Pins the warm-start MLP oracle (oracle/mlp_oracle.cpp): its FP64 forward pass against the same
-network assembled from torch's float64 linear layers and SiLU and against closed forms (zero
-weights, a single path through the layers, SiLU's identity limit); its analytic gradient against
-torch autograd and central finite differences; its BF16 rounding against the round-to-nearest-even
+network assembled from torch's float64 linear layers and hard-swish and against closed forms (zero
+weights, a single path through the layers, hard-swish's identity above 3 and zero below -3); its
+analytic gradient against torch autograd and central finite differences; its BF16 rounding against the round-to-nearest-even
 definition; and its Adam step against the closed form of the first step."""
-import math
 
 import numpy as np
 import pytest
@@ -40,7 +39,9 @@ def test_gradient_matches_finite_differences():
     loss, g = mlp_loss_grad(p, x, t, bf16=False)
     idx = np.concatenate([rng.choice(np.arange(s.start, s.stop), size=min(12, s.stop - s.start), replace=False) for s in MLP_SLICES.values()])
     for i in idx:
-        h = 1e-3 * max(1.0, abs(float(p[i])))
+        # hard-swish has kinks at -3 and 3; a small step keeps the central difference from
+        # straddling one (the loss is a piecewise polynomial, exact to rounding between kinks)
+        h = 1e-5 * max(1.0, abs(float(p[i])))
         pp = p.astype(np.float64).copy()
         # perturb in float32 storage exactly: use the representable neighbours
         p1 = p.copy()
@@ -83,7 +84,7 @@ def test_adam_first_step_closed_form():
 # ---- the forward pass against library routines and closed forms ----------------------------
 def torch_net(p):
     """The network as the packing defines it (W row-major [out][in], oracle/mlp_oracle.cpp),
-    built from torch's float64 linear layers and SiLU."""
+    built from torch's float64 linear layers and hard-swish."""
     import torch
 
     t = torch.tensor(np.asarray(p, np.float64), requires_grad=True)
@@ -95,9 +96,9 @@ def torch_net(p):
 
     def f(x):
         F = torch.nn.functional
-        h1 = F.silu(F.linear(x, W1, t[S["b1"]]))
-        h2 = F.silu(F.linear(h1, W2, t[S["b2"]]))
-        h3 = F.silu(F.linear(h2, W3, t[S["b3"]]))
+        h1 = F.hardswish(F.linear(x, W1, t[S["b1"]]))
+        h2 = F.hardswish(F.linear(h1, W2, t[S["b2"]]))
+        h3 = F.hardswish(F.linear(h2, W3, t[S["b3"]]))
         return F.linear(h3, W4, t[S["b4"]])[:, 0]
 
     return t, f
@@ -105,7 +106,7 @@ def torch_net(p):
 
 def test_forward_matches_torch_float64():
     """Layer order, weight layout and activation: the oracle's FP64 forward equals the same
-    network assembled from torch.nn.functional.linear / silu."""
+    network assembled from torch.nn.functional.linear / hardswish."""
     import torch
 
     rng = np.random.default_rng(11)
@@ -136,8 +137,9 @@ def test_gradient_matches_torch_autograd():
 
 def test_forward_closed_forms():
     """All-zero weights give the output bias; with one path through the network (one unit per
-    layer) the output is silu(silu(silu(w x0 + c))) with silu(z) = z / (1 + e^-z), since silu(0) = 0
-    keeps every other unit silent; and silu(z) -> z for large z."""
+    layer) the output is a(a(a(w x0 + c))) with hard-swish a(z) = z min(max(z + 3, 0), 6) / 6
+    (a(1) = 2/3, a(-1) = -1/3), since a(0) = 0 keeps every other unit silent; a(z) = z above 3 and
+    0 below -3."""
     n = orc().mlp_param_count()
     x = np.zeros((3, 16), np.float32)
     x[:, 0] = [0.5, -1.25, 2.0]
@@ -145,8 +147,10 @@ def test_forward_closed_forms():
     p[MLP_SLICES["b4"]] = 0.375
     np.testing.assert_array_equal(mlp_forward(p, x, bf16=False), [0.375] * 3)
 
-    def silu(z):
-        return z / (1.0 + math.exp(-z))
+    def hswish(z):
+        return z * min(max(z + 3.0, 0.0), 6.0) / 6.0
+
+    assert hswish(1.0) == pytest.approx(2.0 / 3.0) and hswish(-1.0) == pytest.approx(-1.0 / 3.0)
 
     w, c = 1.5, -0.25
     p = np.zeros(n, np.float32)
@@ -155,7 +159,7 @@ def test_forward_closed_forms():
     p[MLP_SLICES["W2"].start] = 1.0  # W2[0][0]
     p[MLP_SLICES["W3"].start] = 1.0  # W3[0][0]
     p[MLP_SLICES["W4"].start] = 1.0  # W4[0]
-    want = [silu(silu(silu(w * float(x0) + c))) for x0 in x[:, 0]]
+    want = [hswish(hswish(hswish(w * float(x0) + c))) for x0 in x[:, 0]]
     np.testing.assert_allclose(mlp_forward(p, x, bf16=False), want, rtol=1e-14)
     # a unit in the second slot of layer 1 that W2 reads from its column 1 (W2[0][1])
     p = np.zeros(n, np.float32)
@@ -165,9 +169,9 @@ def test_forward_closed_forms():
     p[MLP_SLICES["W4"].start] = 1.0
     x2 = np.zeros((2, 16), np.float32)
     x2[:, 3] = [0.75, -2.0]
-    want = [silu(silu(silu(w * float(v)))) for v in x2[:, 3]]
+    want = [hswish(hswish(hswish(w * float(v)))) for v in x2[:, 3]]
     np.testing.assert_allclose(mlp_forward(p, x2, bf16=False), want, rtol=1e-14)
-    # large pre-activations: silu is the identity to double precision
+    # pre-activations above 3: hard-swish is the identity; below -3 it is 0
     p = np.zeros(n, np.float32)
     p[MLP_SLICES["W1"].start] = 1.0
     p[MLP_SLICES["W2"].start] = 1.0
@@ -176,3 +180,5 @@ def test_forward_closed_forms():
     x3 = np.zeros((1, 16), np.float32)
     x3[0, 0] = 60.0
     assert mlp_forward(p, x3, bf16=False)[0] == pytest.approx(60.0, rel=1e-20, abs=1e-12)
+    x3[0, 0] = -7.5
+    assert mlp_forward(p, x3, bf16=False)[0] == 0.0
