@@ -92,6 +92,24 @@ __device__ void load_features(const MlpModel* __restrict__ M, Src src, bf16* Xs,
     }
 }
 
+// Stores one thread's 4 raw features of sample n (valid) standardised, clipped and in BF16, as
+// load_features does; inference fetches the raw values a chunk ahead.
+__device__ __forceinline__ void store_features(float4 raw, bool valid, const float* mean, const float* inv, bf16* Xs,
+                                               bf16* XsT) {
+    const int t = threadIdx.x;
+    const int n = t >> 2, f0 = (t & 3) * 4;
+    const float r[4] = {raw.x, raw.y, raw.z, raw.w};
+    for (int q = 0; q < 4; ++q) {
+        int f = f0 + q;
+        float v = valid ? (r[q] - mean[q]) * inv[q] : 0.0f;
+        v = fminf(fmaxf(v, -8.0f), 8.0f);
+        bf16 b = (bf16)v;
+        Xs[n * LD16 + f] = b;
+        XsT[f * LD64 + n] = b;
+        XsT[(f + 16) * LD64 + n] = (bf16)0.0f;
+    }
+}
+
 // One hidden layer for this wave's tile: z = W[ft rows] * in^T[sh cols] + b.  Writes
 // bf16(act(z)) sample-major into out[64][LD64] and, if outT, feature-major into outT[64][LD64].
 __device__ __forceinline__ f32x16 hidden_layer(const bf16* W, int ldw, int K, const float* bias, const bf16* in, int ldin,
@@ -116,9 +134,23 @@ __global__ __launch_bounds__(kB) void k_mlp_infer(const MlpModel* __restrict__ M
     __shared__ __attribute__((aligned(16))) bf16 H2[64 * LD64];
     __shared__ float ypart[2][64];
     const int w = threadIdx.x >> 6, ft = w & 1, sh = w >> 1;
+    // this thread's 4 features of sample threadIdx.x / 4: the standardisation in registers, and
+    // the raw values fetched one chunk ahead so that the load overlaps the previous chunk's layers
+    const int fs = threadIdx.x >> 2, f0 = (threadIdx.x & 3) * 4;
+    float mean[4], inv[4];
+    for (int q = 0; q < 4; ++q) {
+        mean[q] = M->fmean[f0 + q];
+        inv[q] = M->finv[f0 + q];
+    }
+    auto fetch = [&](int base) -> float4 {
+        const int i = base + fs;
+        return i < n ? *reinterpret_cast<const float4*>(feat + (size_t)i * kNumFeatures + f0) : make_float4(0.f, 0.f, 0.f, 0.f);
+    };
+    float4 next = fetch(blockIdx.x * 64);
     for (int base = blockIdx.x * 64; base < n; base += gridDim.x * 64) {
-        load_features(M, [&](int i) -> const float* { return base + i < n ? feat + (size_t)(base + i) * kNumFeatures : nullptr; },
-                      Xs, XsT);
+        const float4 cur = next;
+        next = fetch(base + gridDim.x * 64);
+        store_features(cur, base + fs < n, mean, inv, Xs, XsT);
         __syncthreads();
         hidden_layer(M->W1, 16, 16, M->b1, Xs, LD16, H1, nullptr, ft, sh);
         __syncthreads();
