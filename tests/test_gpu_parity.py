@@ -320,6 +320,35 @@ def test_neighbour_cap_values(cap, block):
     compare_step(g, o)
 
 
+def test_neighbour_loops_end_at_each_wavefronts_longest_list():
+    """The neighbour passes compute up to their wavefront's largest list length rounded up to
+    even, not to the end of its last group of 8.  A sparse layer over a graded block gives
+    wavefronts (64 consecutive slots) whose largest count takes every value mod 8, from inside the
+    first group of 8 up to the sixth; two steps are bit for bit the oracle's."""
+    rng = np.random.default_rng(5)
+    sc = scenes.dam_break((16, 16, 16), (0, 0, 0, 1.6, 1.6, 1.6), seed=7)
+    u = rng.uniform(size=(sc.n, 3))
+    pos = np.empty((sc.n, 3), np.float32)
+    pos[:, 0] = 0.05 + 1.5 * u[:, 0] ** 0.25
+    pos[:, 1] = 0.05 + 1.5 * u[:, 1]
+    pos[:, 2] = 0.05 + 0.75 * u[:, 2]
+    pos[:600, 2] = 0.85 + 0.7 * u[:600, 2]  # the sparse layer
+    sc.pos = pos
+    g, o = pair(sc)
+    g.step()
+    o.step(1)
+    _, _, _, srt, nin = o.grid()
+    _, cnt = o.neighbors()
+    c = cnt[srt[:nin]]
+    wmax = {int(c[i : i + 64].max()) for i in range(0, nin, 64)}
+    assert {m % 8 for m in wmax} == set(range(8))
+    assert min(wmax) < 8 and max(wmax) > 40
+    compare_step(g, o)
+    g.step()
+    o.step(1)
+    compare_step(g, o)
+
+
 def test_shuffled_input_order():
     """Particles given in random order: in the first step each workgroup of the scatter holds
     particles of many scan tiles (16^3 cells + 1 = 5 tiles of 1024 counters), which it scans one
