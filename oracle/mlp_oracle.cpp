@@ -8,8 +8,8 @@
 //          -> z3 = W3 h2 + b3 (64) -> h3 = bf16(act(z3))
 //          -> y  = W4 h3 + b4 (1)
 //
-// act is hard-swish, z * min(max(z + 3, 0), 6) / 6, with torch's derivative at the kinks (0 below
-// -3, z / 3 + 1/2 from -3 up to and including 3, 1 above).
+// act is hard-swish, z * min(max(z + 3, 0), 6) / 6, with torch's derivative at the kinks (0 up to
+// and including -3, z / 3 + 1/2 between, 1 from 3 on).
 //
 // Weights W are row-major [out][in].  Inputs x and weights are BF16 on the device; products are
 // accumulated in FP32 there and in FP64 here.  bf16() is round-to-nearest-even of the FP32
@@ -39,7 +39,7 @@ inline float bf16_round(float f) {
 }
 
 inline double act(double z) { return z * std::min(std::max(z + 3.0, 0.0), 6.0) / 6.0; }
-inline double dact(double z) { return z < -3.0 ? 0.0 : (z <= 3.0 ? z / 3.0 + 0.5 : 1.0); }
+inline double dact(double z) { return z <= -3.0 ? 0.0 : (z < 3.0 ? z / 3.0 + 0.5 : 1.0); }
 
 struct Params {
     const float *W1, *b1, *W2, *b2, *W3, *b3, *W4, *b4;

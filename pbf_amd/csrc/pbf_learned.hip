@@ -36,12 +36,12 @@ constexpr int LD16 = 24;  // ... of 16-wide rows
 // Hard-swish, h = z * clamp(z + 3, 0, 6) / 6 (A. Howard et al., "Searching for MobileNetV3",
 // 2019): SiLU's shape from full-rate operations (an add, a clamp, two multiplies).  SiLU's exp and
 // reciprocal issue at a quarter of the rate and took about half of the inference kernel's issue
-// cycles.  The derivative is torch's: 0 below -3, z / 3 + 1/2 up to 3, 1 above.
+// cycles.  The derivative is torch's: 0 up to and including -3, z / 3 + 1/2 between, 1 from 3 on.
 __device__ __forceinline__ float actf(float z) {
     return (z * __builtin_amdgcn_fmed3f(z + 3.0f, 0.0f, 6.0f)) * (1.0f / 6.0f);
 }
 __device__ __forceinline__ float dactf(float z) {
-    return z < -3.0f ? 0.0f : (z <= 3.0f ? z * (1.0f / 3.0f) + 0.5f : 1.0f);
+    return z <= -3.0f ? 0.0f : (z < 3.0f ? z * (1.0f / 3.0f) + 0.5f : 1.0f);
 }
 
 // acc += A[32 x K] * B[32 x K]^T, rows of A at a + r*lda, rows of B at b + r*ldb (bf16,

@@ -182,3 +182,27 @@ def test_forward_closed_forms():
     assert mlp_forward(p, x3, bf16=False)[0] == pytest.approx(60.0, rel=1e-20, abs=1e-12)
     x3[0, 0] = -7.5
     assert mlp_forward(p, x3, bf16=False)[0] == 0.0
+
+
+def test_gradient_at_the_kinks_follows_torch():
+    """Hard-swish's derivative where it is not differentiable: a single path through the network
+    whose first pre-activation is exactly -3, 0, 3 or 4 (the other layers' identity weights keep
+    the later pre-activations away from the kinks or at 0).  The oracle's gradient equals torch's
+    autograd, which takes 0 up to and including -3, z / 3 + 1/2 between and 1 from 3 on."""
+    import torch
+
+    n = orc().mlp_param_count()
+    p = np.zeros(n, np.float32)
+    p[MLP_SLICES["W1"].start] = 1.0  # W1[0][0]
+    p[MLP_SLICES["W2"].start] = 1.0
+    p[MLP_SLICES["W3"].start] = 1.0
+    p[MLP_SLICES["W4"].start] = 1.0
+    for z0 in (-3.0, 0.0, 3.0, 4.0):
+        x = np.zeros((1, 16), np.float32)
+        x[0, 0] = z0
+        t = np.array([0.5], np.float32)
+        tp, f = torch_net(p)
+        y = f(torch.tensor(x.astype(np.float64)))
+        ((y - torch.tensor(t.astype(np.float64))) ** 2).mean().backward()
+        _, g = mlp_loss_grad(p, x, t, bf16=False)
+        np.testing.assert_allclose(g, tp.grad.numpy(), rtol=1e-12, atol=1e-14, err_msg=f"z = {z0}")

@@ -43,7 +43,7 @@ MUTANTS = [
     ("hswish_shift", "mlp_oracle.cpp", "return z * std::min(std::max(z + 3.0, 0.0), 6.0) / 6.0;",
      "return z * std::min(std::max(z + 2.0, 0.0), 6.0) / 6.0;",
      [MLP + "test_forward_matches_torch_float64", MLP + "test_forward_closed_forms"]),
-    ("hswish_slope", "mlp_oracle.cpp", "(z <= 3.0 ? z / 3.0 + 0.5 : 1.0)", "(z <= 3.0 ? z / 6.0 + 0.5 : 1.0)",
+    ("hswish_slope", "mlp_oracle.cpp", "(z < 3.0 ? z / 3.0 + 0.5 : 1.0)", "(z < 3.0 ? z / 6.0 + 0.5 : 1.0)",
      [MLP + "test_gradient_matches_torch_autograd", MLP + "test_gradient_matches_finite_differences"]),
     ("w2_transposed", "mlp_oracle.cpp", "z += w(P.W2[o * H + i]) * a1[i];", "z += w(P.W2[i * H + o]) * a1[i];",
      [MLP + "test_forward_matches_torch_float64", MLP + "test_forward_closed_forms"]),
