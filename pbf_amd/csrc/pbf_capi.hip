@@ -59,6 +59,7 @@ struct pbf_solver {
     int nstride = 0;
     int cap = 0;
     int device = 0;
+    int cus = 256;  // compute units of the device
     hipStream_t stream = nullptr;
     StepBuffers B{};
     int nbuckets = 0;            // cell counters: ncells + 1, or the sum over a batch
@@ -145,6 +146,8 @@ StepConsts make_consts(const pbf_solver* s) {
     K.rmin = P.grad_rmin;
     K.rmin2 = P.grad_rmin * P.grad_rmin;
     K.fast = (P.flags & PBF_FLAG_FAST_MATH) ? 1 : 0;
+    // at most one wavefront per SIMD: the neighbour passes load list entries a group ahead
+    K.lpf = (int64_t)s->n <= (int64_t)64 * 4 * s->cus ? 1 : 0;
     for (int a = 0; a < 3; ++a) {
         K.g[a] = P.gravity[a];
         K.gmin[a] = s->gs[a];
@@ -505,6 +508,13 @@ int create_solver(const pbf_params* para, const float* grid_space, const float* 
     pbf_solver* s = new pbf_solver();
     s->P = *para;
     s->P.nParticles = n;
+    {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+            prop.multiProcessorCount > 0)
+            s->cus = prop.multiProcessorCount;
+    }
     std::memcpy(s->gs, grid_space, sizeof(s->gs));
     std::memcpy(s->bd, boundary, sizeof(s->bd));
     s->n = n;

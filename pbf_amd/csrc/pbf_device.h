@@ -35,6 +35,7 @@ struct StepConsts {
     float h, h2, c6, cs, wdq, inv_wdq, k, scorr_n, restRho, eps, dt, visc_c, vort_eps, jitter, rmin;
     float rmin2;  // rmin * rmin (the fast mode's cut-off test)
     int fast;     // PBF_FLAG_FAST_MATH: the constraint kernels' FAST instances
+    int lpf;      // the neighbour passes' PF instances (list entries loaded a group ahead)
     int n_int;  // exponent of s_corr when it is a small non-negative integer, else -1
     float g[3];
     float gmin[3];

@@ -443,16 +443,28 @@ __device__ __forceinline__ void run_group(int rem, G&& g) {
 // Calls f(qb, x_q) for the list entries j = 0 .. min(ceil8(cnt), ceil2(cmax)) - 1 of slot s in
 // order, qb = 16 q the entry's byte offset and x_q = pos[q] (entries past cnt are the padding
 // slot), cmax the wavefront's largest count.
-template <class F>
+// PF (grids of at most one wavefront per SIMD, where every memory round trip shows): the next
+// group's entries load while the current group is summed, issued after the gathers and without a
+// branch (the last group reloads itself), so that the wait for the gathers leaves them in flight.
+// With more wavefronts the others hide the round trip and the prefetch costs time.
+template <bool PF, class F>
 __device__ __forceinline__ void for_neighbors(const uint32_t* __restrict__ nbr, int s, int nstride, int cnt,
                                               const float4* __restrict__ pos, F&& f) {
     const int cmax = wave_max_count(cnt);
+    uint32_t qn[kGroup];
+    if (PF && cnt > 0) load_group(nbr, s, 0, nstride, qn);
     for (int j0 = 0; j0 < cnt; j0 += kGroup) {
         uint32_t q[kGroup];
-        load_group(nbr, s, j0, nstride, q);
+        if constexpr (PF) {
+#pragma unroll
+            for (int u = 0; u < kGroup; ++u) q[u] = qn[u];
+        } else {
+            load_group(nbr, s, j0, nstride, q);
+        }
         float4 x[kGroup];
 #pragma unroll
         for (int u = 0; u < kGroup; ++u) x[u] = slot_at(pos, q[u]);
+        if constexpr (PF) load_group(nbr, s, min(j0 + kGroup, (cnt - 1) & ~(kGroup - 1)), nstride, qn);
         // The gathers are used here so that the compiler issues all of them before the tail
         // branches below instead of sinking each into its branch behind a wait of its own.
         asm volatile("" ::"v"(x[0].x), "v"(x[1].x), "v"(x[2].x), "v"(x[3].x), "v"(x[4].x), "v"(x[5].x),
@@ -640,7 +652,7 @@ __global__ __launch_bounds__(BS) void k_search(StepConsts K, const SceneGeom* __
 // to pos.w and to lam1 (a warm-start feature of pass 2).
 // With L.feat, C at the predicted positions goes to feature slot 10; with L.resid the workgroup
 // sums of max(C, 0) and |C| at the predicted positions.
-template <int BS, bool FAST>
+template <int BS, bool PF, bool FAST>
 __global__ __launch_bounds__(BS) void k_lambda_first(StepConsts K, float4* __restrict__ pos,
                                                      const uint32_t* __restrict__ nbr, const uint8_t* __restrict__ ncnt,
                                                      float* __restrict__ lam1, float* __restrict__ dbg_lam,
@@ -654,7 +666,7 @@ __global__ __launch_bounds__(BS) void k_lambda_first(StepConsts K, float4* __res
         const int cnt = ncnt[s];
         LambdaAcc A;
         A.init();
-        for_neighbors(nbr, s, K.nstride, cnt, pos, [&](uint32_t, const float4& xj) {
+        for_neighbors<PF>(nbr, s, K.nstride, cnt, pos, [&](uint32_t, const float4& xj) {
             float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
             A.template add<FAST>(rx, ry, rz, dot3f(rx, ry, rz), K);
         });
@@ -672,7 +684,7 @@ __global__ __launch_bounds__(BS) void k_lambda_first(StepConsts K, float4* __res
 
 // calculateLambda (cu:156-180) over the frozen list, iterations 2..I.  With L.resid, the
 // workgroup sums of max(C_i, 0) and |C_i| at the current positions are written as well.
-template <int BS, bool FEAT, bool FAST>
+template <int BS, bool FEAT, bool PF, bool FAST>
 __global__ __launch_bounds__(BS) void k_lambda(StepConsts K, const SceneGeom* __restrict__ geoms,
                                                const uint8_t* __restrict__ sid, float4* __restrict__ pos,
                                                const uint32_t* __restrict__ nbr, const uint8_t* __restrict__ ncnt,
@@ -687,7 +699,7 @@ __global__ __launch_bounds__(BS) void k_lambda(StepConsts K, const SceneGeom* __
         int cnt = ncnt[s];
         LambdaAcc A;
         A.init();
-        for_neighbors(nbr, s, K.nstride, cnt, pos, [&](uint32_t, const float4& xj) {
+        for_neighbors<PF>(nbr, s, K.nstride, cnt, pos, [&](uint32_t, const float4& xj) {
             float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
             A.template add<FAST>(rx, ry, rz, dot3f(rx, ry, rz), K);
         });
@@ -738,7 +750,7 @@ __global__ __launch_bounds__(BS) void k_lambda(StepConsts K, const SceneGeom* __
 // LS: the multipliers come from LA.lam_src (n + 1 entries, the padding slot's 0 included)
 // instead of pos.w: the fallback pass after a rejected learned guess reads the kept lambda 2
 // there, so pos.w (the guess) needs no restore.
-template <int BS, int SN, bool LS, bool FAST>
+template <int BS, int SN, bool LS, bool PF, bool FAST>
 __global__ __launch_bounds__(BS) void k_delta_apply(StepConsts K, const SceneGeom* __restrict__ geoms,
                                                     const uint8_t* __restrict__ sid, const float4* __restrict__ pos,
                                                     float4* __restrict__ pos_next, const uint32_t* __restrict__ nbr,
@@ -755,7 +767,7 @@ __global__ __launch_bounds__(BS) void k_delta_apply(StepConsts K, const SceneGeo
         int cnt = ncnt[s];
         if (LA.lam_sum) LA.lam_sum[s] = (first ? 0.0f : LA.lam_sum[s]) + lam_i;
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
-        for_neighbors(nbr, s, K.nstride, cnt, pos, [&](uint32_t q, const float4& xj) {
+        for_neighbors<PF>(nbr, s, K.nstride, cnt, pos, [&](uint32_t q, const float4& xj) {
             float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
             float r2 = dot3f(rx, ry, rz);
             float w = poly6(r2, K);
@@ -875,7 +887,7 @@ __global__ __launch_bounds__(BS) void k_curl(StepConsts K, float4* __restrict__ 
 // (cu:4-13, 31-40) on the final state, which is in registers here: predicted cell and bucket
 // place per slot, counted into the cell counters (the scan of this step has cleared them).  The
 // next step's reorder predicts again from the same values, so both agree bit for bit.
-template <int BS, bool HASH, bool FAST>
+template <int BS, bool HASH, bool PF, bool FAST>
 __global__ __launch_bounds__(BS) void k_vort_apply(StepConsts K, const SceneGeom* __restrict__ geoms,
                                                    const uint8_t* __restrict__ sid, const float4* __restrict__ pos,
                                                    const float4* __restrict__ vel, const float4* __restrict__ omega,
@@ -893,7 +905,7 @@ __global__ __launch_bounds__(BS) void k_vort_apply(StepConsts K, const SceneGeom
         const float4 p = xsph[s];
         int cnt = ncnt[s];
         float ex = 0.0f, ey = 0.0f, ez = 0.0f;
-        for_neighbors(nbr, s, K.nstride, cnt, pos, [&](uint32_t, const float4& xj) {
+        for_neighbors<PF>(nbr, s, K.nstride, cnt, pos, [&](uint32_t, const float4& xj) {
             const float wl = xj.w;  // |omega_j|
             float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
             float r2 = dot3f(rx, ry, rz);
@@ -1105,13 +1117,35 @@ void launch_sort_reorder(hipStream_t st, const StepConsts& K, const Geometry& GE
 }
 
 // The FAST instances (hardware rsqrt in the spiky gradient) when the step constants ask for them.
-#define PBF_DISPATCH_FAST(K, CALL)                 \
+#define PBF_DISPATCH_FAST_ONLY(K, CALL)            \
     if ((K).fast) {                                \
         constexpr bool FAST = true;                \
         CALL;                                      \
     } else {                                       \
         constexpr bool FAST = false;               \
         CALL;                                      \
+    }
+
+// The neighbour passes' instances by (K).fast and (K).lpf: FAST and PF for CALL.
+#define PBF_DISPATCH_FAST(K, CALL)                 \
+    if ((K).fast) {                                \
+        constexpr bool FAST = true;                \
+        if ((K).lpf) {                             \
+            constexpr bool PF = true;              \
+            CALL;                                  \
+        } else {                                   \
+            constexpr bool PF = false;             \
+            CALL;                                  \
+        }                                          \
+    } else {                                       \
+        constexpr bool FAST = false;               \
+        if ((K).lpf) {                             \
+            constexpr bool PF = true;              \
+            CALL;                                  \
+        } else {                                   \
+            constexpr bool PF = false;             \
+            CALL;                                  \
+        }                                          \
     }
 
 #define PBF_DISPATCH_BS(T, CALL)            \
@@ -1145,7 +1179,7 @@ void launch_neighbors_lambda(hipStream_t st, const StepConsts& K, const Geometry
         }
     }
 #undef PBF_SEARCH
-    PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_lambda_first<BS, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, B.nbr,
+    PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_lambda_first<BS, PF, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, B.nbr,
                                           B.ncnt, B.lam1, dbg_lam, L, T.xcd_map)));
 }
 
@@ -1153,10 +1187,10 @@ void launch_lambda(hipStream_t st, const StepConsts& K, const Geometry& GE, cons
                    const StepBuffers& B, const LearnArgs& L) {
     if (K.n <= 0) return;
     if (L.feat) {
-        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_lambda<BS, true, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
+        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_lambda<BS, true, PF, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
                                               GE.sid, pos, B.nbr, B.ncnt, L, T.xcd_map)));
     } else {
-        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_lambda<BS, false, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
+        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_lambda<BS, false, PF, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
                                               GE.sid, pos, B.nbr, B.ncnt, L, T.xcd_map)));
     }
 }
@@ -1165,15 +1199,15 @@ void launch_delta_apply(hipStream_t st, const StepConsts& K, const Geometry& GE,
                         float4* pos_next, float4* vel, const StepBuffers& B, int last, int first, const LearnArgs& L) {
     if (K.n <= 0) return;
     if (L.lam_src) {  // the fallback after a rejected learned guess (rare): the generic s_corr kernel
-        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_delta_apply<BS, 0, true, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K,
+        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_delta_apply<BS, 0, true, PF, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K,
                                               GE.geoms, GE.sid, pos, pos_next, B.nbr, B.ncnt, B.xold, vel, last, first,
                                               L, T.xcd_map)));
     } else if (K.n_int == 4) {
-        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_delta_apply<BS, 4, false, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K,
+        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_delta_apply<BS, 4, false, PF, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K,
                                               GE.geoms, GE.sid, pos, pos_next, B.nbr, B.ncnt, B.xold, vel, last, first,
                                               L, T.xcd_map)));
     } else {
-        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_delta_apply<BS, 0, false, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K,
+        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_delta_apply<BS, 0, false, PF, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K,
                                               GE.geoms, GE.sid, pos, pos_next, B.nbr, B.ncnt, B.xold, vel, last, first,
                                               L, T.xcd_map)));
     }
@@ -1183,10 +1217,10 @@ void launch_curl(hipStream_t st, const StepConsts& K, const Tuning& T, float4* p
                  const StepBuffers& B, float* err, float* partial) {
     if (K.n <= 0) return;
     if (err || partial) {
-        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_curl<BS, true, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, vel,
+        PBF_DISPATCH_FAST_ONLY(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_curl<BS, true, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, vel,
                                               B.nbr, B.ncnt, B.omega, B.xold, err, partial, T.xcd_map)));
     } else {
-        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_curl<BS, false, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, vel,
+        PBF_DISPATCH_FAST_ONLY(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_curl<BS, false, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, pos, vel,
                                               B.nbr, B.ncnt, B.omega, B.xold, err, partial, T.xcd_map)));
     }
 }
@@ -1195,11 +1229,11 @@ void launch_vort_apply(hipStream_t st, const StepConsts& K, const Geometry& GE, 
                        const float4* vel, float4* vel_out, const StepBuffers& B, bool hash_next) {
     if (K.n <= 0) return;
     if (hash_next) {
-        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_vort_apply<BS, true, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
+        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_vort_apply<BS, true, PF, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K, GE.geoms,
                                               GE.sid, pos, vel, B.omega, B.xold, B.nbr, B.ncnt, vel_out, B.cell, B.local,
                                               B.count, T.xcd_map)));
     } else {
-        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_vort_apply<BS, false, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K,
+        PBF_DISPATCH_FAST(K, PBF_DISPATCH_BS(T, hipLaunchKernelGGL((k_vort_apply<BS, false, PF, FAST>), dim3(ngrid(T, K.n)), dim3(BS), 0, st, K,
                                               GE.geoms, GE.sid, pos, vel, B.omega, B.xold, B.nbr, B.ncnt, vel_out, B.cell,
                                               B.local, B.count, T.xcd_map)));
     }
